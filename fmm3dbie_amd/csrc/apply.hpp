@@ -1,0 +1,58 @@
+// Device-resident apply plan (apply.hip): everything lpcomp_*_addsub needs, uploaded
+// once and reused across matvecs (GMRES) and right-hand sides.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <vector>
+
+enum { APPLY_ADDSUB = 0, APPLY_MASKED = 1 };
+
+struct ApplyPlanInput {
+  bool helm;
+  int64_t npatches, npts, nptso, nnz, nquad;
+  const int64_t *norders, *ixyzs, *novers, *ixyzso;  // host
+  const int64_t *row_ptr, *col_ind, *iquad;          // host, 1-based
+  const double *srcvals, *srcover, *whtsover;        // host
+  const double* wnear;    // complex (interleaved) if helm, else real
+  bool wnear_is_device;   // wnear already in device memory
+  double zk[2], alpha[2], beta[2];
+};
+
+struct ApplyTimes {
+  double ms_over, ms_far, ms_csr, ms_sub;
+};
+
+struct ApplyPlan {
+  bool helm = true;
+  int64_t npatches = 0, npts = 0, nptso = 0, nnz = 0, nquad = 0;
+  double zk[2] = {0, 0}, alpha[2] = {0, 0}, beta[2] = {0, 0};
+  double thresh = 0;
+  int uniform_p = 0;
+  int64_t *ixyzs = nullptr, *ixyzso = nullptr, *row_ptr = nullptr, *col_ind = nullptr, *iquad = nullptr,
+          *ioff = nullptr;
+  int *np_of = nullptr, *opatch = nullptr, *near_sorted = nullptr;
+  double* imat = nullptr;
+  double* wover = nullptr;
+  float* wover32 = nullptr;
+  double* src64[6] = {};
+  float* src32[6] = {};
+  double* trg64[3] = {};
+  float* trg32[3] = {};
+  double* wnear64 = nullptr;  // complex128 (Laplace weights with zero imaginary part)
+  float* wnear32 = nullptr;   // complex64 copy for the mixed-precision path
+  // workspace
+  int ws_cap = 0;
+  double *q64 = nullptr, *m64 = nullptr, *sig64c = nullptr, *pot64c = nullptr;
+  float *q32 = nullptr, *m32 = nullptr;
+  std::vector<void*> owned;
+  ~ApplyPlan();
+};
+
+int apply_plan_build(ApplyPlan* P, const ApplyPlanInput& in, hipStream_t s);
+// prec 64: complex128 sigma/pot; prec 32: complex64 (always masked)
+int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* pot, int nrhs, hipStream_t s,
+                   ApplyTimes* tm);
+int apply_plan_run_real(ApplyPlan* P, int mode, const double* sigma, double* pot, hipStream_t s);
+void apply_plan_free_ws(ApplyPlan* P);

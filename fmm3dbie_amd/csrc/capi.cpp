@@ -1,0 +1,574 @@
+// C-ABI of libfmm3dbie_amd.so (declared in include/fmm3dbie_amd.h).
+//
+// * Reference-signature entry points (gfortran mangling, every argument by pointer,
+//   int64 1-based index arrays, column-major host arrays) -- the drop-in for
+//   getnearquad_{helm,lap}_comb_dir(_eval) and lpcomp_{helm,lap}_comb_dir_addsub.
+//   They copy the host arrays to HBM, run the gfx950 kernels and copy the result back.
+// * Device-pointer entry points (fmm3dbie_*_dev / apply plans) used by the Python layer
+//   and the benchmark: inputs already resident in HBM, work on a caller stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "apply.hpp"
+#include "gmres.hpp"
+#include "host_basis.hpp"
+#include "quad.hpp"
+
+namespace {
+
+int g_last_error = 0;
+
+template <typename T>
+T* upload(const T* h, size_t n, hipStream_t s) {
+  void* p = nullptr;
+  if (hipMalloc(&p, (n ? n : 1) * sizeof(T)) != hipSuccess) return nullptr;
+  if (n && hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess) return nullptr;
+  return (T*)p;
+}
+
+int uniform_order(int64_t npatches, const int64_t* norders, const int64_t* ixyzs, const int64_t* iptype) {
+  for (int64_t i = 0; i < npatches; i++) {
+    if (iptype && iptype[i] != 1) {
+      fprintf(stderr, "fmm3dbie_amd: only triangular RV patches (iptype=1) are supported\n");
+      return -7;
+    }
+    if (norders[i] != norders[0]) {
+      fprintf(stderr, "fmm3dbie_amd: patches of different orders are not supported\n");
+      return -8;
+    }
+    if (ixyzs[i + 1] - ixyzs[i] != fmmhost::npols((int)norders[i])) {
+      fprintf(stderr, "fmm3dbie_amd: ixyzs inconsistent with norders\n");
+      return -9;
+    }
+  }
+  return (int)norders[0];
+}
+
+int norder_avg(int64_t npatches, const int64_t* norders) {
+  int64_t s = 0;
+  for (int64_t i = 0; i < npatches; i++) s += norders[i];
+  return (int)std::floor(s / (npatches + 0.0));
+}
+
+// Host-array near quadrature: shared by the four getnearquad_* symbols.
+int nearquad_host(bool helm, int64_t npatches, const int64_t* norders, const int64_t* ixyzs,
+                  const int64_t* iptype, int64_t npts, const double* srccoefs, const double* srcvals,
+                  int64_t ndtarg, int64_t ntarg, const double* targs, const int64_t* ipatch_id,
+                  const double* uvs_targ, double eps, const double* kpar6, int64_t nnz, const int64_t* row_ptr,
+                  const int64_t* col_ind, const int64_t* iquad, double rfac0, int64_t nquad, double* wnear) {
+  const int no = uniform_order(npatches, norders, ixyzs, iptype);
+  if (no < 0) return no;
+  hipStream_t s;
+  if (hipStreamCreate(&s) != hipSuccess) return -1;
+  std::vector<void*> bufs;
+  auto up = [&](auto* h, size_t n) {
+    auto* d = upload(h, n, s);
+    bufs.push_back((void*)d);
+    return d;
+  };
+  NearQuadArgs A = {};
+  A.helm = helm;
+  A.norder = no;
+  A.norder_branch = norder_avg(npatches, norders);
+  A.npatches = npatches;
+  A.ixyzs = up(ixyzs, npatches + 1);
+  A.srccoefs = up(srccoefs, 9 * npts);
+  A.srcvals = up(srcvals, 12 * npts);
+  A.ndtarg = ndtarg;
+  A.ntarg = ntarg;
+  A.targs = up(targs, ndtarg * ntarg);
+  A.ipatch_id = up(ipatch_id, ntarg);
+  A.uvs_targ = up(uvs_targ, 2 * ntarg);
+  A.eps = eps;
+  A.zk[0] = kpar6[0]; A.zk[1] = kpar6[1];
+  A.alpha[0] = kpar6[2]; A.alpha[1] = kpar6[3];
+  A.beta[0] = kpar6[4]; A.beta[1] = kpar6[5];
+  A.nnz = nnz;
+  A.row_ptr = up(row_ptr, ntarg + 1);
+  A.col_ind = up(col_ind, nnz);
+  A.iquad = up(iquad, nnz + 1);
+  A.rfac0 = rfac0;
+  const size_t wn = (helm ? 2 : 1) * (size_t)nquad;
+  double* dw = nullptr;
+  int rc = 0;
+  if (hipMalloc((void**)&dw, (wn ? wn : 1) * sizeof(double)) != hipSuccess) rc = -2;
+  bufs.push_back(dw);
+  for (void* b : bufs)
+    if (!b) rc = -2;
+  if (rc == 0) {
+    (void)hipMemsetAsync(dw, 0, wn * sizeof(double), s);
+    A.wnear = dw;
+    A.self_n = 10;
+    A.self_tau_panel = 1.0;
+    rc = fmm_nearquad_device(A, nullptr, s);
+  }
+  if (rc == 0 && hipMemcpyAsync(wnear, dw, wn * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) rc = -1;
+  if (hipStreamSynchronize(s) != hipSuccess && rc == 0) rc = -1;
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  (void)hipStreamDestroy(s);
+  g_last_error = rc;
+  return rc;
+}
+
+// get_patch_id_uvs (surf_routs.f90:1736-1762): patch id and RV node (u,v) per point
+void patch_id_uvs(int64_t npatches, const int64_t* norders, const int64_t* ixyzs, std::vector<int64_t>& pid,
+                  std::vector<double>& uvs) {
+  const int64_t npts = ixyzs[npatches] - 1;
+  pid.assign(npts, -1);
+  uvs.assign(2 * npts, 0.0);
+  for (int64_t ip = 0; ip < npatches; ip++) {
+    for (int64_t j = ixyzs[ip] - 1; j < ixyzs[ip + 1] - 1; j++) pid[j] = ip + 1;
+    fmmhost::rv_nodes((int)norders[ip], uvs.data() + 2 * (ixyzs[ip] - 1), nullptr);
+  }
+}
+
+int lpcomp_host(bool helm, int64_t npatches, const int64_t* norders, const int64_t* ixyzs, const int64_t* iptype,
+                int64_t npts, const double* srcvals, const double* kpar6, int64_t nnz, const int64_t* row_ptr,
+                const int64_t* col_ind, const int64_t* iquad, int64_t nquad, const double* wnear,
+                const int64_t* novers, int64_t nptso, const int64_t* ixyzso, const double* srcover,
+                const double* whtsover, const double* sigma, double* pot) {
+  for (int64_t i = 0; i < npatches; i++)
+    if (iptype && iptype[i] != 1) {
+      fprintf(stderr, "fmm3dbie_amd: only triangular RV patches (iptype=1) are supported\n");
+      return -7;
+    }
+  hipStream_t s;
+  if (hipStreamCreate(&s) != hipSuccess) return -1;
+  ApplyPlanInput in = {};
+  in.helm = helm;
+  in.npatches = npatches; in.npts = npts; in.nptso = nptso; in.nnz = nnz; in.nquad = nquad;
+  in.norders = norders; in.ixyzs = ixyzs; in.novers = novers; in.ixyzso = ixyzso;
+  in.row_ptr = row_ptr; in.col_ind = col_ind; in.iquad = iquad;
+  in.srcvals = srcvals; in.srcover = srcover; in.whtsover = whtsover;
+  in.wnear = wnear; in.wnear_is_device = false;
+  in.zk[0] = kpar6[0]; in.zk[1] = kpar6[1];
+  in.alpha[0] = kpar6[2]; in.alpha[1] = kpar6[3];
+  in.beta[0] = kpar6[4]; in.beta[1] = kpar6[5];
+  int rc = 0;
+  {
+    ApplyPlan P;
+    rc = apply_plan_build(&P, in, s);
+    const size_t n = (helm ? 2 : 1) * (size_t)npts;
+    double *ds = nullptr, *dp = nullptr;
+    if (rc == 0 && (hipMalloc((void**)&ds, n * 8) != hipSuccess || hipMalloc((void**)&dp, n * 8) != hipSuccess))
+      rc = -2;
+    if (rc == 0 && hipMemcpyAsync(ds, sigma, n * 8, hipMemcpyHostToDevice, s) != hipSuccess) rc = -1;
+    if (rc == 0)
+      rc = helm ? apply_plan_run(&P, 64, APPLY_ADDSUB, ds, dp, 1, s, nullptr)
+                : apply_plan_run_real(&P, APPLY_ADDSUB, ds, dp, s);
+    if (rc == 0 && hipMemcpyAsync(pot, dp, n * 8, hipMemcpyDeviceToHost, s) != hipSuccess) rc = -1;
+    if (hipStreamSynchronize(s) != hipSuccess && rc == 0) rc = -1;
+    if (ds) (void)hipFree(ds);
+    if (dp) (void)hipFree(dp);
+  }
+  (void)hipStreamDestroy(s);
+  g_last_error = rc;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmm3dbie_last_error() { return g_last_error; }
+
+// ---------------------------------------------------------------- reference drop-ins
+
+void getnearquad_helm_comb_dir_eval_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                                     const int64_t* iptype, const int64_t* npts, const double* srccoefs,
+                                     const double* srcvals, const int64_t* ndtarg, const int64_t* ntarg,
+                                     const double* targs, const int64_t* ipatch_id, const double* uvs_targ,
+                                     const double* eps, const double* zpars, const int64_t* iquadtype,
+                                     const int64_t* nnz, const int64_t* row_ptr, const int64_t* col_ind,
+                                     const int64_t* iquad, const double* rfac0, const int64_t* nquad,
+                                     double* wnear) {
+  if (*iquadtype != 1) {
+    fprintf(stderr, "fmm3dbie_amd: iquadtype %ld not supported\n", (long)*iquadtype);
+    g_last_error = -10;
+    return;
+  }
+  nearquad_host(true, *npatches, norders, ixyzs, iptype, *npts, srccoefs, srcvals, *ndtarg, *ntarg, targs,
+                ipatch_id, uvs_targ, *eps, zpars, *nnz, row_ptr, col_ind, iquad, *rfac0, *nquad, wnear);
+}
+
+void getnearquad_helm_comb_dir_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                                const int64_t* iptype, const int64_t* npts, const double* srccoefs,
+                                const double* srcvals, const double* eps, const double* zpars,
+                                const int64_t* iquadtype, const int64_t* nnz, const int64_t* row_ptr,
+                                const int64_t* col_ind, const int64_t* iquad, const double* rfac0,
+                                const int64_t* nquad, double* wnear) {
+  std::vector<int64_t> pid;
+  std::vector<double> uvs;
+  patch_id_uvs(*npatches, norders, ixyzs, pid, uvs);
+  const int64_t nd = 12;
+  getnearquad_helm_comb_dir_eval_(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, &nd, npts, srcvals,
+                                  pid.data(), uvs.data(), eps, zpars, iquadtype, nnz, row_ptr, col_ind, iquad,
+                                  rfac0, nquad, wnear);
+}
+
+void getnearquad_lap_comb_dir_eval_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                                    const int64_t* iptype, const int64_t* npts, const double* srccoefs,
+                                    const double* srcvals, const int64_t* ndtarg, const int64_t* ntarg,
+                                    const double* targs, const int64_t* ipatch_id, const double* uvs_targ,
+                                    const double* eps, const double* dpars, const int64_t* iquadtype,
+                                    const int64_t* nnz, const int64_t* row_ptr, const int64_t* col_ind,
+                                    const int64_t* iquad, const double* rfac0, const int64_t* nquad,
+                                    double* wnear) {
+  if (*iquadtype != 1) {
+    fprintf(stderr, "fmm3dbie_amd: iquadtype %ld not supported\n", (long)*iquadtype);
+    g_last_error = -10;
+    return;
+  }
+  const double kp[6] = {0.0, 0.0, dpars[0], 0.0, dpars[1], 0.0};
+  nearquad_host(false, *npatches, norders, ixyzs, iptype, *npts, srccoefs, srcvals, *ndtarg, *ntarg, targs,
+                ipatch_id, uvs_targ, *eps, kp, *nnz, row_ptr, col_ind, iquad, *rfac0, *nquad, wnear);
+}
+
+void getnearquad_lap_comb_dir_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                               const int64_t* iptype, const int64_t* npts, const double* srccoefs,
+                               const double* srcvals, const double* eps, const double* dpars,
+                               const int64_t* iquadtype, const int64_t* nnz, const int64_t* row_ptr,
+                               const int64_t* col_ind, const int64_t* iquad, const double* rfac0,
+                               const int64_t* nquad, double* wnear) {
+  std::vector<int64_t> pid;
+  std::vector<double> uvs;
+  patch_id_uvs(*npatches, norders, ixyzs, pid, uvs);
+  const int64_t nd = 12;
+  getnearquad_lap_comb_dir_eval_(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, &nd, npts, srcvals,
+                                 pid.data(), uvs.data(), eps, dpars, iquadtype, nnz, row_ptr, col_ind, iquad,
+                                 rfac0, nquad, wnear);
+}
+
+void lpcomp_helm_comb_dir_addsub_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                                  const int64_t* iptype, const int64_t* npts, const double* srccoefs,
+                                  const double* srcvals, const double* eps, const int64_t* ndd,
+                                  const double* dpars, const int64_t* ndz, const double* zpars,
+                                  const int64_t* ndi, const int64_t* ipars, const int64_t* nnz,
+                                  const int64_t* row_ptr, const int64_t* col_ind, const int64_t* iquad,
+                                  const int64_t* nquad, const int64_t* nker, const double* wnear,
+                                  const int64_t* novers, const int64_t* nptso, const int64_t* ixyzso,
+                                  const double* srcover, const double* whtsover, const int64_t* lwork,
+                                  const double* work, const int64_t* ndim, const double* sigma, double* pot) {
+  (void)srccoefs; (void)eps; (void)ndd; (void)dpars; (void)ndz; (void)ndi; (void)ipars; (void)lwork; (void)work;
+  if (*nker != 1 || *ndim != 1) {
+    fprintf(stderr, "fmm3dbie_amd: nker=%ld ndim=%ld not supported (1, 1)\n", (long)*nker, (long)*ndim);
+    g_last_error = -11;
+    return;
+  }
+  lpcomp_host(true, *npatches, norders, ixyzs, iptype, *npts, srcvals, zpars, *nnz, row_ptr, col_ind, iquad,
+              *nquad, wnear, novers, *nptso, ixyzso, srcover, whtsover, sigma, pot);
+}
+
+void lpcomp_lap_comb_dir_addsub_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                                 const int64_t* iptype, const int64_t* npts, const double* srccoefs,
+                                 const double* srcvals, const double* eps, const int64_t* ndd,
+                                 const double* dpars, const int64_t* ndz, const double* zpars,
+                                 const int64_t* ndi, const int64_t* ipars, const int64_t* nnz,
+                                 const int64_t* row_ptr, const int64_t* col_ind, const int64_t* iquad,
+                                 const int64_t* nquad, const int64_t* nker, const double* wnear,
+                                 const int64_t* novers, const int64_t* nptso, const int64_t* ixyzso,
+                                 const double* srcover, const double* whtsover, const int64_t* lwork,
+                                 const double* work, const int64_t* ndim, const double* sigma, double* pot) {
+  (void)srccoefs; (void)eps; (void)ndd; (void)ndz; (void)zpars; (void)ndi; (void)ipars; (void)lwork; (void)work;
+  if (*nker != 1 || *ndim != 1) {
+    fprintf(stderr, "fmm3dbie_amd: nker=%ld ndim=%ld not supported (1, 1)\n", (long)*nker, (long)*ndim);
+    g_last_error = -11;
+    return;
+  }
+  const double kp[6] = {0.0, 0.0, dpars[0], 0.0, dpars[1], 0.0};
+  lpcomp_host(false, *npatches, norders, ixyzs, iptype, *npts, srcvals, kp, *nnz, row_ptr, col_ind, iquad, *nquad,
+              wnear, novers, *nptso, ixyzso, srcover, whtsover, sigma, pot);
+}
+
+// get_iquad_rsc (near_field_routs.f:765-820)
+void get_iquad_rsc_(const int64_t* npatches, const int64_t* ixyzs, const int64_t* npts, const int64_t* nnz,
+                    const int64_t* row_ptr, const int64_t* col_ind, int64_t* iquad) {
+  (void)npatches; (void)npts; (void)row_ptr;
+  iquad[0] = 1;
+  for (int64_t i = 0; i < *nnz; i++) {
+    const int64_t ip = col_ind[i] - 1;
+    iquad[i + 1] = iquad[i] + (ixyzs[ip + 1] - ixyzs[ip]);
+  }
+}
+
+// get_near_corr_max (near_field_routs.f:998-1057)
+void get_near_corr_max_(const int64_t* ntarg, const int64_t* row_ptr, const int64_t* nnz, const int64_t* col_ind,
+                        const int64_t* npatches, const int64_t* ixyzs, int64_t* nmax) {
+  (void)nnz; (void)npatches;
+  int64_t m = 0;
+  for (int64_t i = 0; i < *ntarg; i++) {
+    int64_t t = 0;
+    for (int64_t j = row_ptr[i] - 1; j < row_ptr[i + 1] - 1; j++) t += ixyzs[col_ind[j]] - ixyzs[col_ind[j] - 1];
+    if (t > m) m = t;
+  }
+  *nmax = m;
+}
+
+// get_fmm_thresh (get_fmm_thresh.f:1-90)
+void get_fmm_thresh_(const int64_t* nds, const int64_t* ns, const double* src, const int64_t* ndt,
+                     const int64_t* nt, const double* trg, double* thresh) {
+  double mn[3], mx[3];
+  for (int m = 0; m < 3; m++) mn[m] = mx[m] = src[m];
+  for (int64_t i = 0; i < *ns; i++)
+    for (int m = 0; m < 3; m++) {
+      const double v = src[*nds * i + m];
+      mn[m] = v < mn[m] ? v : mn[m];
+      mx[m] = v > mx[m] ? v : mx[m];
+    }
+  for (int64_t i = 0; i < *nt; i++)
+    for (int m = 0; m < 3; m++) {
+      const double v = trg[*ndt * i + m];
+      mn[m] = v < mn[m] ? v : mn[m];
+      mx[m] = v > mx[m] ? v : mx[m];
+    }
+  double b = mx[0] - mn[0];
+  if (mx[1] - mn[1] > b) b = mx[1] - mn[1];
+  if (mx[2] - mn[2] > b) b = mx[2] - mn[2];
+  *thresh = std::ldexp(1.0, -51) * b;
+}
+
+// rsc_to_csc (sparse_reps.f:38-72): stable in row order; iper maps CSC -> CSR index
+void rsc_to_csc_(const int64_t* ncol, const int64_t* nrow, const int64_t* nnz, const int64_t* row_ptr,
+                 const int64_t* col_ind, int64_t* col_ptr, int64_t* row_ind, int64_t* iper) {
+  std::vector<int64_t> cnt(*ncol + 1, 0), fill(*ncol, 0);
+  for (int64_t j = 0; j < *nnz; j++) cnt[col_ind[j]]++;
+  col_ptr[0] = 1;
+  for (int64_t i = 0; i < *ncol; i++) col_ptr[i + 1] = col_ptr[i] + cnt[i + 1];
+  for (int64_t t = 0; t < *nrow; t++)
+    for (int64_t j = row_ptr[t] - 1; j < row_ptr[t + 1] - 1; j++) {
+      const int64_t c = col_ind[j] - 1;
+      const int64_t pos = col_ptr[c] - 1 + fill[c]++;
+      row_ind[pos] = t + 1;
+      iper[pos] = j + 1;
+    }
+}
+
+// ---------------------------------------------------------------- device-pointer API
+
+// kpar6 = (k re, k im, alpha re, alpha im, beta re, beta im); Laplace: helm=0, k ignored.
+// stats (optional, 12 doubles): nself, nadap, nfixed, overflow, nf, kq, ms_prep, ms_fixed,
+// ms_adap, ms_self, 0, 0.
+int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatches, const int64_t* ixyzs,
+                          const double* srccoefs, const double* srcvals, int64_t ndtarg, int64_t ntarg,
+                          const double* targs, const int64_t* ipatch_id, const double* uvs_targ, double eps,
+                          const double* kpar6, int64_t nnz, const int64_t* row_ptr, const int64_t* col_ind,
+                          const int64_t* iquad, double rfac0, double* wnear, int self_n, double self_tau_panel,
+                          void* stream, double* stats, int* nref_host, int* class_host) {
+  NearQuadArgs A = {};
+  A.helm = helm != 0;
+  A.norder = norder;
+  A.norder_branch = norder_branch;
+  A.npatches = npatches;
+  A.ixyzs = ixyzs; A.srccoefs = srccoefs; A.srcvals = srcvals;
+  A.ndtarg = ndtarg; A.ntarg = ntarg; A.targs = targs; A.ipatch_id = ipatch_id; A.uvs_targ = uvs_targ;
+  A.eps = eps;
+  A.zk[0] = kpar6[0]; A.zk[1] = kpar6[1];
+  A.alpha[0] = kpar6[2]; A.alpha[1] = kpar6[3];
+  A.beta[0] = kpar6[4]; A.beta[1] = kpar6[5];
+  A.nnz = nnz; A.row_ptr = row_ptr; A.col_ind = col_ind; A.iquad = iquad;
+  A.rfac0 = rfac0; A.wnear = wnear;
+  A.self_n = self_n > 0 ? self_n : 10;
+  A.self_tau_panel = self_tau_panel > 0 ? self_tau_panel : 1.0;
+  A.want_nref = nref_host != nullptr;
+  A.nref_host = nref_host;
+  A.class_host = class_host;
+  NearQuadStats st = {};
+  const int rc = fmm_nearquad_device(A, &st, (hipStream_t)stream);
+  if (stats) {
+    const double v[12] = {(double)st.nself, (double)st.nadap, (double)st.nfixed, (double)st.overflow,
+                          (double)st.nf, (double)st.kq, st.ms_prep, st.ms_fixed, st.ms_adap, st.ms_self, 0, 0};
+    std::memcpy(stats, v, sizeof(v));
+  }
+  g_last_error = rc;
+  return rc;
+}
+
+void* fmm3dbie_apply_plan_create(int helm, int64_t npatches, const int64_t* norders, const int64_t* ixyzs,
+                                 int64_t npts, const double* srcvals, const double* kpar6, int64_t nnz,
+                                 const int64_t* row_ptr, const int64_t* col_ind, const int64_t* iquad,
+                                 int64_t nquad, const double* wnear, int wnear_on_device, const int64_t* novers,
+                                 int64_t nptso, const int64_t* ixyzso, const double* srcover,
+                                 const double* whtsover, void* stream) {
+  ApplyPlanInput in = {};
+  in.helm = helm != 0;
+  in.npatches = npatches; in.npts = npts; in.nptso = nptso; in.nnz = nnz; in.nquad = nquad;
+  in.norders = norders; in.ixyzs = ixyzs; in.novers = novers; in.ixyzso = ixyzso;
+  in.row_ptr = row_ptr; in.col_ind = col_ind; in.iquad = iquad;
+  in.srcvals = srcvals; in.srcover = srcover; in.whtsover = whtsover;
+  in.wnear = wnear; in.wnear_is_device = wnear_on_device != 0;
+  in.zk[0] = kpar6[0]; in.zk[1] = kpar6[1];
+  in.alpha[0] = kpar6[2]; in.alpha[1] = kpar6[3];
+  in.beta[0] = kpar6[4]; in.beta[1] = kpar6[5];
+  ApplyPlan* P = new ApplyPlan();
+  const int rc = apply_plan_build(P, in, (hipStream_t)stream);
+  g_last_error = rc;
+  if (rc != 0) {
+    delete P;
+    return nullptr;
+  }
+  return P;
+}
+
+// prec 64 (complex128) or 32 (complex64); mode 0 add/subtract, 1 masked (prec 32 is
+// always masked); times (optional, 4 doubles): oversample, far, csr, subtract ms.
+int fmm3dbie_apply_plan_apply(void* plan, int prec, int mode, const void* sigma, void* pot, int nrhs, void* stream,
+                              double* times) {
+  if (!plan) return -12;
+  ApplyTimes tm = {};
+  const int rc = apply_plan_run((ApplyPlan*)plan, prec, mode, sigma, pot, nrhs, (hipStream_t)stream,
+                                times ? &tm : nullptr);
+  if (times) {
+    times[0] = tm.ms_over; times[1] = tm.ms_far; times[2] = tm.ms_csr; times[3] = tm.ms_sub;
+  }
+  g_last_error = rc;
+  return rc;
+}
+
+int fmm3dbie_apply_plan_info(void* plan, int64_t* out4) {
+  if (!plan) return -12;
+  ApplyPlan* P = (ApplyPlan*)plan;
+  out4[0] = P->npts; out4[1] = P->nptso; out4[2] = P->nnz; out4[3] = P->uniform_p;
+  return 0;
+}
+
+void fmm3dbie_apply_plan_destroy(void* plan) { delete (ApplyPlan*)plan; }
+
+// GMRES on a device-resident plan.  stats (optional, 2 ints): applies, FP64 applies.
+int fmm3dbie_gmres_dev(void* plan, int prec, int mode, const double* zid, const void* rhs, void* soln,
+                       const double* wts, int nrhs, int numit, double eps_gmres, int* niter, double* errs,
+                       double* rres, void* stream, int* stats) {
+  if (!plan) return -12;
+  GmresStats gs;
+  const int rc = gmres_device((ApplyPlan*)plan, prec, mode, zid, rhs, soln, wts, nrhs, numit, eps_gmres, niter,
+                              errs, rres, (hipStream_t)stream, &gs);
+  if (stats) {
+    stats[0] = gs.napply;
+    stats[1] = gs.napply64;
+  }
+  g_last_error = rc;
+  return rc;
+}
+
+int fmm3dbie_gmres_ir_dev(void* plan, int mode, const double* zid, const void* rhs, void* soln, const double* wts,
+                          int nrhs, int numit, double eps_inner, double tol, int max_outer, int* outer,
+                          int* inner_total, double* rres, void* stream, int* stats) {
+  if (!plan) return -12;
+  GmresStats gs;
+  const int rc = gmres_ir_device((ApplyPlan*)plan, mode, zid, rhs, soln, wts, nrhs, numit, eps_inner, tol,
+                                 max_outer, outer, inner_total, rres, (hipStream_t)stream, &gs);
+  if (stats) {
+    stats[0] = gs.napply;
+    stats[1] = gs.napply64;
+  }
+  g_last_error = rc;
+  return rc;
+}
+
+// helm_comb_dir_solver_guru (helm_comb_dir.f:2071-2229) / lap_comb_dir_solver_guru
+// (lap_comb_dir.f90:1459-1650) with host arrays: zid = -(-1)^ifinout * beta/2.
+static int solver_guru_host(bool helm, int64_t npatches, const int64_t* norders, const int64_t* ixyzs,
+                            const int64_t* iptype, int64_t npts, const double* srcvals, const double* kp6,
+                            int64_t numit, int64_t ifinout, const double* rhs, int64_t nnz, const int64_t* row_ptr,
+                            const int64_t* col_ind, const int64_t* iquad, int64_t nquad, const double* wnear,
+                            const int64_t* novers, int64_t nptso, const int64_t* ixyzso, const double* srcover,
+                            const double* wover, double eps_gmres, int64_t* niter, double* errs, double* rres,
+                            double* soln) {
+  hipStream_t s;
+  if (hipStreamCreate(&s) != hipSuccess) return -1;
+  ApplyPlanInput in = {};
+  in.helm = helm;
+  in.npatches = npatches; in.npts = npts; in.nptso = nptso; in.nnz = nnz; in.nquad = nquad;
+  in.norders = norders; in.ixyzs = ixyzs; in.novers = novers; in.ixyzso = ixyzso;
+  in.row_ptr = row_ptr; in.col_ind = col_ind; in.iquad = iquad;
+  in.srcvals = srcvals; in.srcover = srcover; in.whtsover = wover;
+  in.wnear = wnear; in.wnear_is_device = false;
+  in.zk[0] = kp6[0]; in.zk[1] = kp6[1]; in.alpha[0] = kp6[2]; in.alpha[1] = kp6[3];
+  in.beta[0] = kp6[4]; in.beta[1] = kp6[5];
+  (void)iptype;
+  int rc;
+  {
+    ApplyPlan P;
+    rc = apply_plan_build(&P, in, s);
+    // quadrature weights |x_u x x_v| w_RV (get_qwts)
+    std::vector<double> w(npts), rvw(300);
+    for (int64_t ip = 0; ip < npatches && rc == 0; ip++) {
+      fmmhost::rv_nodes((int)norders[ip], nullptr, rvw.data());
+      for (int64_t j = ixyzs[ip] - 1; j < ixyzs[ip + 1] - 1; j++) {
+        const double* v = srcvals + 12 * j;
+        const double nx = v[4] * v[8] - v[5] * v[7], ny = v[5] * v[6] - v[3] * v[8], nz = v[3] * v[7] - v[4] * v[6];
+        w[j] = std::sqrt(nx * nx + ny * ny + nz * nz) * rvw[j - (ixyzs[ip] - 1)];
+      }
+    }
+    std::vector<double> rc2(2 * npts);
+    for (int64_t i = 0; i < npts; i++) {
+      rc2[2 * i] = helm ? rhs[2 * i] : rhs[i];
+      rc2[2 * i + 1] = helm ? rhs[2 * i + 1] : 0.0;
+    }
+    double *dw = nullptr, *dr = nullptr, *dx = nullptr;
+    if (rc == 0 && (hipMalloc((void**)&dw, 8 * npts) != hipSuccess || hipMalloc((void**)&dr, 16 * npts) != hipSuccess ||
+                    hipMalloc((void**)&dx, 16 * npts) != hipSuccess))
+      rc = -2;
+    if (rc == 0) {
+      (void)hipMemcpyAsync(dw, w.data(), 8 * npts, hipMemcpyHostToDevice, s);
+      (void)hipMemcpyAsync(dr, rc2.data(), 16 * npts, hipMemcpyHostToDevice, s);
+      const double sgn = (ifinout % 2 == 0) ? -1.0 : 1.0;  // -(-1)^ifinout
+      const double zid[2] = {sgn * kp6[4] / 2, sgn * kp6[5] / 2};
+      int ni = 0;
+      std::vector<double> e(numit);
+      rc = gmres_device(&P, 64, APPLY_ADDSUB, zid, dr, dx, dw, 1, (int)numit, eps_gmres, &ni, e.data(), rres, s,
+                        nullptr);
+      *niter = ni;
+      for (int i = 0; i < ni; i++) errs[i] = e[i];
+      (void)hipMemcpyAsync(rc2.data(), dx, 16 * npts, hipMemcpyDeviceToHost, s);
+      (void)hipStreamSynchronize(s);
+      for (int64_t i = 0; i < npts; i++) {
+        if (helm) { soln[2 * i] = rc2[2 * i]; soln[2 * i + 1] = rc2[2 * i + 1]; }
+        else soln[i] = rc2[2 * i];
+      }
+    }
+    if (dw) (void)hipFree(dw);
+    if (dr) (void)hipFree(dr);
+    if (dx) (void)hipFree(dx);
+  }
+  (void)hipStreamDestroy(s);
+  g_last_error = rc;
+  return rc;
+}
+
+void helm_comb_dir_solver_guru_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                                const int64_t* iptype, const int64_t* npts, const double* srccoefs,
+                                const double* srcvals, const double* eps, const double* zpars, const int64_t* numit,
+                                const int64_t* ifinout, const double* rhs, const int64_t* nnz,
+                                const int64_t* row_ptr, const int64_t* col_ind, const int64_t* iquad,
+                                const int64_t* nquad, const int64_t* nker, const double* wnear,
+                                const int64_t* novers, const int64_t* npts_over, const int64_t* ixyzso,
+                                const double* srcover, const double* wover, const double* eps_gmres,
+                                int64_t* niter, double* errs, double* rres, double* soln) {
+  (void)srccoefs; (void)eps; (void)nker;
+  solver_guru_host(true, *npatches, norders, ixyzs, iptype, *npts, srcvals, zpars, *numit, *ifinout, rhs, *nnz,
+                   row_ptr, col_ind, iquad, *nquad, wnear, novers, *npts_over, ixyzso, srcover, wover, *eps_gmres,
+                   niter, errs, rres, soln);
+}
+
+void lap_comb_dir_solver_guru_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                               const int64_t* iptype, const int64_t* npts, const double* srccoefs,
+                               const double* srcvals, const double* eps, const double* dpars, const int64_t* numit,
+                               const int64_t* ifinout, const double* rhs, const int64_t* nnz,
+                               const int64_t* row_ptr, const int64_t* col_ind, const int64_t* iquad,
+                               const int64_t* nquad, const int64_t* nker, const double* wnear,
+                               const int64_t* novers, const int64_t* nptso, const int64_t* ixyzso,
+                               const double* srcover, const double* whtsover, const double* eps_gmres,
+                               int64_t* niter, double* errs, double* rres, double* soln) {
+  (void)srccoefs; (void)eps; (void)nker;
+  const double kp[6] = {0.0, 0.0, dpars[0], 0.0, dpars[1], 0.0};
+  solver_guru_host(false, *npatches, norders, ixyzs, iptype, *npts, srcvals, kp, *numit, *ifinout, rhs, *nnz,
+                   row_ptr, col_ind, iquad, *nquad, wnear, novers, *nptso, ixyzso, srcover, whtsover, *eps_gmres,
+                   niter, errs, rres, soln);
+}
+
+}  // extern "C"

@@ -1,0 +1,692 @@
+// Near-field quadrature generation on gfx950 (getnearquad_{helm,lap}_comb_dir*).
+//
+// Pipeline (all on one stream, inputs resident in HBM):
+//   1. k_patch_prep   : per patch centroid/radius of the vertex images
+//                       (get_centroid_rads, surf_routs.f90:195-302) and the adaptive
+//                       tolerance epsp = eps_adap*rsc^(1/4) (ggq-quads.f:329-337).
+//   2. k_classify     : per CSR entry -> self / adaptive (|x-c| <= rfac0*rad) / fixed
+//                       (ggq-quads.f:341-365, get_near_far_split_pt surf_routs.f90:1653).
+//   3. k_fixed        : one workgroup per source patch; the patch's oversampled rule
+//                       geometry is built once in LDS and every far-near target of the
+//                       patch is integrated against it (ctriaints_wnodes,
+//                       triaintrouts.f:67-271 + umat^T, ggq-quads.f:412).
+//   4. k_adaptive     : persistent wavefronts pull (target, patch) pairs from a queue
+//                       and run the 1->4 refinement of triaadap_main
+//                       (triaintrouts.f:1269-1464) depth-first with a per-wave stack in
+//                       HBM; 64 lanes evaluate the XG nodes of one child at a time.
+//   5. k_self         : one wavefront per on-surface target; a polar/sinh product rule
+//                       on the three triangles meeting at the target replaces the GGQ
+//                       rule of zget_ggq_self_quad_pt (ggq-quads.f:933-1122).
+// Moments are formed by staging (f_q, P_j(q)) rows in LDS and letting lane c sum
+// column c, then weights = umat^T * moments.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "common.hpp"
+#include "host_basis.hpp"
+#include "quad.hpp"
+
+__constant__ KoornConst g_koorn;
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int STACK_CAP = 72;
+constexpr int NLMAX = 20;
+
+struct GLRule {
+  double x[16];
+  double w[16];
+};
+__constant__ GLRule g_self_gl;
+
+template <bool HELM>
+__device__ __forceinline__ void kern_eval(const double* tg, const double* s, const KPar& kp, double& vr,
+                                          double& vi) {
+  if (HELM) {
+    helm_comb(tg[0] - s[0], tg[1] - s[1], tg[2] - s[2], s[9], s[10], s[11], kp, vr, vi);
+  } else {
+    vr = lap_comb(tg[0] - s[0], tg[1] - s[1], tg[2] - s[2], s[9], s[10], s[11], kp);
+    vi = 0.0;
+  }
+}
+
+// ------------------------------------------------------------------ 1. patch prep
+template <int N>
+__global__ void k_patch_prep(int64_t npatches, const int64_t* __restrict__ ixyzs,
+                             const double* __restrict__ srccoefs, const double* __restrict__ srcvals,
+                             const double* __restrict__ rvw, double eps_adap, double* __restrict__ cms,
+                             double* __restrict__ rads, double* __restrict__ epsp) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  const int64_t ip = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (ip >= npatches) return;
+  const int64_t i0 = ixyzs[ip] - 1;
+  const double* sc = srccoefs + 9 * i0;
+  double P[NP], xyz[3][3];
+  const double vu[3] = {0.0, 1.0, 0.0}, vv[3] = {0.0, 0.0, 1.0};
+  for (int k = 0; k < 3; k++) {
+    koorn_eval<N>(vu[k], vv[k], P);
+    for (int m = 0; m < 3; m++) {
+      double s = 0.0;
+      for (int l = 0; l < NP; l++) s += sc[9 * l + m] * P[l];
+      xyz[k][m] = s;
+    }
+  }
+  double c[3];
+  for (int m = 0; m < 3; m++) c[m] = (xyz[0][m] + xyz[1][m] + xyz[2][m]) / 3;
+  double r = 0.0;
+  for (int k = 0; k < 3; k++) {
+    const double d = (c[0] - xyz[k][0]) * (c[0] - xyz[k][0]) + (c[1] - xyz[k][1]) * (c[1] - xyz[k][1]) +
+                     (c[2] - xyz[k][2]) * (c[2] - xyz[k][2]);
+    if (d >= r) r = d;
+  }
+  cms[3 * ip] = c[0];
+  cms[3 * ip + 1] = c[1];
+  cms[3 * ip + 2] = c[2];
+  rads[ip] = sqrt(r);
+  double rsc = 0.0;
+  for (int i = 0; i < NP; i++) {
+    const double* s = srcvals + 12 * (i0 + i);
+    const double nx = s[4] * s[8] - s[5] * s[7], ny = s[5] * s[6] - s[3] * s[8],
+                 nz = s[3] * s[7] - s[4] * s[6];
+    const double rr = (nx * nx + ny * ny + nz * nz) * rvw[i] * rvw[i];
+    if (i == 0 || rr < rsc) rsc = rr;
+  }
+  epsp[ip] = eps_adap * pow(rsc, 0.25);
+}
+
+// ------------------------------------------------------------------ 2. classify
+__global__ void k_classify(int64_t ntarg, int64_t ndtarg, const double* __restrict__ targs,
+                           const int64_t* __restrict__ ipatch_id, const int64_t* __restrict__ row_ptr,
+                           const int64_t* __restrict__ col_ind, const double* __restrict__ cms,
+                           const double* __restrict__ rads, double rfac0, int* __restrict__ fixed_cnt,
+                           int2* __restrict__ adap_list, int* __restrict__ adap_n,
+                           int2* __restrict__ self_list, int* __restrict__ self_n,
+                           int* __restrict__ pair_class) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ntarg) return;
+  const double tx = targs[ndtarg * t], ty = targs[ndtarg * t + 1], tz = targs[ndtarg * t + 2];
+  const int64_t pid = ipatch_id ? ipatch_id[t] : -1;
+  for (int64_t j = row_ptr[t] - 1; j < row_ptr[t + 1] - 1; j++) {
+    const int64_t ip = col_ind[j] - 1;
+    int cls;
+    if (pid == ip + 1) {
+      cls = 0;
+      const int k = atomicAdd(self_n, 1);
+      self_list[k] = make_int2((int)j, (int)t);
+    } else {
+      const double dx = tx - cms[3 * ip], dy = ty - cms[3 * ip + 1], dz = tz - cms[3 * ip + 2];
+      const double rr = rfac0 * rads[ip];
+      if (dx * dx + dy * dy + dz * dz <= rr * rr) {
+        cls = 1;
+        const int k = atomicAdd(adap_n, 1);
+        adap_list[k] = make_int2((int)j, (int)t);
+      } else {
+        cls = 2;
+        atomicAdd(fixed_cnt + ip, 1);
+      }
+    }
+    if (pair_class) pair_class[j] = cls;
+  }
+}
+
+__global__ void k_scan(int64_t n, const int* __restrict__ cnt, int* __restrict__ off) {
+  // single workgroup exclusive scan (n = number of patches)
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int64_t chunk = (n + blockDim.x - 1) / blockDim.x;
+  const int64_t a = tid * chunk, b = a + chunk < n ? a + chunk : n;
+  int s = 0;
+  for (int64_t i = a; i < b; i++) s += cnt[i];
+  part[tid] = s;
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int i = 0; i < (int)blockDim.x; i++) {
+      const int v = part[i];
+      part[i] = acc;
+      acc += v;
+    }
+    off[n] = acc;
+  }
+  __syncthreads();
+  int acc = part[tid];
+  for (int64_t i = a; i < b; i++) {
+    off[i] = acc;
+    acc += cnt[i];
+  }
+}
+
+__global__ void k_scatter_fixed(int64_t ntarg, const int64_t* __restrict__ row_ptr,
+                                const int64_t* __restrict__ col_ind, const int* __restrict__ pair_class,
+                                const int* __restrict__ off, int* __restrict__ fill,
+                                int2* __restrict__ fixed_list) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ntarg) return;
+  for (int64_t j = row_ptr[t] - 1; j < row_ptr[t + 1] - 1; j++) {
+    if (pair_class[j] != 2) continue;
+    const int64_t ip = col_ind[j] - 1;
+    const int k = atomicAdd(fill + ip, 1);
+    fixed_list[off[ip] + k] = make_int2((int)j, (int)t);
+  }
+}
+
+// ------------------------------------------------------------------ 3. fixed rule
+// Bq[q*NP + j] = w_q * sum_l P_l(q) umat(l,j); Pq[q*NP + l] = P_l(q); quv[2q] = (u,v).
+template <int N, bool HELM>
+__global__ void __launch_bounds__(128) k_fixed(int nf, const double* __restrict__ quv,
+                                               const double* __restrict__ Pq, const double* __restrict__ Bq,
+                                               const int64_t* __restrict__ ixyzs,
+                                               const double* __restrict__ srccoefs, int64_t ndtarg,
+                                               const double* __restrict__ targs,
+                                               const int64_t* __restrict__ iquad, const int* __restrict__ off,
+                                               const int2* __restrict__ fixed_list, KPar kp,
+                                               double* __restrict__ wnear) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  extern __shared__ double lds[];
+  const int ip = blockIdx.x;
+  const int a = off[ip], b = off[ip + 1];
+  if (a == b) return;
+  double* sc = lds;              // 9*NP
+  double* node = lds + 9 * NP;   // nf * 7 : xyz, n, |J|
+  const double* gsc = srccoefs + 9 * (ixyzs[ip] - 1);
+  for (int i = threadIdx.x; i < 9 * NP; i += blockDim.x) sc[i] = gsc[i];
+  __syncthreads();
+  for (int q = threadIdx.x; q < nf; q += blockDim.x) {
+    double P[NP], g[12];
+#pragma unroll
+    for (int l = 0; l < NP; l++) P[l] = Pq[q * NP + l];
+    const double jac = geom_eval<NP>(sc, P, g);
+    double* o = node + 7 * q;
+    o[0] = g[0]; o[1] = g[1]; o[2] = g[2];
+    o[3] = g[9]; o[4] = g[10]; o[5] = g[11];
+    o[6] = jac;
+  }
+  __syncthreads();
+  for (int k = a + threadIdx.x; k < b; k += blockDim.x) {
+    const int2 jt = fixed_list[k];
+    const double* tg = targs + ndtarg * (int64_t)jt.y;
+    const double tx = tg[0], ty = tg[1], tz = tg[2];
+    double ar[NP], ai[NP];
+#pragma unroll
+    for (int j = 0; j < NP; j++) { ar[j] = 0.0; ai[j] = 0.0; }
+#pragma unroll 1
+    for (int q = 0; q < nf; q++) {
+      const double* o = node + 7 * q;
+      double vr, vi;
+      if (HELM) {
+        helm_comb(tx - o[0], ty - o[1], tz - o[2], o[3], o[4], o[5], kp, vr, vi);
+      } else {
+        vr = lap_comb(tx - o[0], ty - o[1], tz - o[2], o[3], o[4], o[5], kp);
+        vi = 0.0;
+      }
+      vr *= o[6];
+      vi *= o[6];
+      const double* bq = Bq + q * NP;
+#pragma unroll
+      for (int j = 0; j < NP; j++) {
+        const double bb = bq[j];
+        ar[j] = fma(vr, bb, ar[j]);
+        if (HELM) ai[j] = fma(vi, bb, ai[j]);
+      }
+    }
+    const int64_t q0 = iquad[jt.x] - 1;
+    if (HELM) {
+#pragma unroll
+      for (int j = 0; j < NP; j++) {
+        wnear[2 * (q0 + j)] = ar[j];
+        wnear[2 * (q0 + j) + 1] = ai[j];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NP; j++) wnear[q0 + j] = ar[j];
+    }
+  }
+}
+
+// Moments of a set of staged nodes: row q of buf holds (f_re, f_im, P_0..P_{NP-1}) of
+// node q (written by the lane that evaluated it); lane c (< 2NP for complex) sums column c.
+template <int NP, bool HELM>
+__device__ __forceinline__ double column_sum(const double* buf, int nrows) {
+  constexpr int W = NP + 2;
+  const int lane = threadIdx.x;
+  __syncthreads();
+  double s = 0.0;
+  constexpr int NC = HELM ? 2 * NP : NP;
+  if (lane < NC) {
+    const int j = lane % NP, part = lane / NP;
+#pragma unroll 4
+    for (int q = 0; q < nrows; q++) s = fma(buf[q * W + part], buf[q * W + 2 + j], s);
+  }
+  __syncthreads();
+  return s;
+}
+
+// Apply umat^T to per-lane moment components and store the weights of one pair.
+template <int NP, bool HELM>
+__device__ __forceinline__ void store_weights(double* buf, double mom, const double* __restrict__ umat,
+                                              double* __restrict__ wnear, int64_t q0) {
+  const int lane = threadIdx.x;
+  constexpr int NC = HELM ? 2 * NP : NP;
+  if (lane < NC) buf[lane] = mom;
+  __syncthreads();
+  if (lane < NC) {
+    const int j = lane % NP, part = lane / NP;
+    double s = 0.0;
+    for (int l = 0; l < NP; l++) s = fma(umat[l + j * NP], buf[part * NP + l], s);
+    if (HELM) wnear[2 * (q0 + j) + part] = s;
+    else wnear[q0 + j] = s;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ 4. adaptive
+// Moments over one tree triangle (x0,y0) + diag(dx,dy)*(u,v) of the XG(kq) rule.
+template <int N, bool HELM>
+__device__ __forceinline__ double tri_moments(double* buf, const double* sc, double x0, double y0, double dx,
+                                              double dy, double da, int kq, const double* __restrict__ xg,
+                                              const double* tg, const KPar& kp) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  constexpr int W = NP + 2;
+  const int lane = threadIdx.x;
+  double total = 0.0;
+  for (int base = 0; base < kq; base += WAVE) {
+    const int q = base + lane;
+    if (q < kq) {
+      double* row = buf + lane * W;
+      const double u = x0 + dx * xg[3 * q], v = y0 + dy * xg[3 * q + 1];
+      double g[12];
+      koorn_geom_row<N>(u, v, sc, g, row + 2);
+      const double jac = geom_finish(g);
+      const double w = jac * da * xg[3 * q + 2];
+      double vr, vi;
+      kern_eval<HELM>(tg, g, kp, vr, vi);
+      row[0] = vr * w;
+      row[1] = vi * w;
+    }
+    const int nrows = kq - base < WAVE ? kq - base : WAVE;
+    total += column_sum<NP, HELM>(buf, nrows);
+  }
+  return total;
+}
+
+template <int N, bool HELM>
+__global__ void __launch_bounds__(64) k_adaptive(int npairs, const int2* __restrict__ list,
+                                                 int* __restrict__ counter, int kq, const double* __restrict__ xg,
+                                                 const int64_t* __restrict__ col_ind,
+                                                 const int64_t* __restrict__ ixyzs,
+                                                 const double* __restrict__ srccoefs, int64_t ndtarg,
+                                                 const double* __restrict__ targs,
+                                                 const int64_t* __restrict__ iquad,
+                                                 const double* __restrict__ epsp_patch,
+                                                 const double* __restrict__ umat, KPar kp,
+                                                 double* __restrict__ stack_all, int* __restrict__ nref_out,
+                                                 int* __restrict__ overflow, double* __restrict__ wnear) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  constexpr int ES = 6 + 2 * NP;  // stack entry: x0,y0,dx,dy,level,pad, moments
+  __shared__ double sc[9 * NP];
+  __shared__ double buf[WAVE * (NP + 2)];
+  __shared__ int sh_idx;
+  const int lane = threadIdx.x;
+  double* stack = stack_all + (size_t)blockIdx.x * STACK_CAP * ES;
+  constexpr int NC = HELM ? 2 * NP : NP;
+  for (;;) {
+    if (lane == 0) sh_idx = atomicAdd(counter, 1);
+    __syncthreads();
+    const int idx = sh_idx;
+    __syncthreads();
+    if (idx >= npairs) break;
+    const int2 jt = list[idx];
+    const int64_t ip = col_ind[jt.x] - 1;
+    const double* gsc = srccoefs + 9 * (ixyzs[ip] - 1);
+    for (int i = lane; i < 9 * NP; i += WAVE) sc[i] = gsc[i];
+    __syncthreads();
+    const double* tp = targs + ndtarg * (int64_t)jt.y;
+    const double tg[3] = {tp[0], tp[1], tp[2]};
+    const double eps = epsp_patch[ip];
+    // root triangle
+    double mroot = tri_moments<N, HELM>(buf, sc, 0.0, 0.0, 1.0, 1.0, 1.0, kq, xg, tg, kp);
+    int sp = 0;
+    if (lane < 5) stack[lane] = lane < 2 ? 0.0 : (lane < 4 ? 1.0 : 0.0);
+    if (lane < NC) stack[6 + lane] = mroot;
+    sp = 1;
+    double result = 0.0;
+    int nref = 0;
+    while (sp > 0) {
+      sp--;
+      const double* e = stack + (size_t)sp * ES;
+      const double x0 = e[0], y0 = e[1], dx = e[2], dy = e[3];
+      const int level = (int)e[4];
+      const double mpar = lane < NC ? e[6 + lane] : 0.0;
+      const double hx = 0.5 * dx, hy = 0.5 * dy, da = hx * hx;  // child area ratio 4^-(level+1)
+      const double cx[4] = {x0, x0 + hx, x0, x0 + hx};
+      const double cy[4] = {y0, y0, y0 + hy, y0 + hy};
+      const double cdx[4] = {hx, hx, hx, -hx};
+      const double cdy[4] = {hy, hy, hy, -hy};
+      double mc[4];
+      double ssum = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        mc[c] = tri_moments<N, HELM>(buf, sc, cx[c], cy[c], cdx[c], cdy[c], da, kq, xg, tg, kp);
+        ssum += mc[c];
+      }
+      nref++;
+      // error = max_j |sum children - parent| (complex modulus)
+      const double dlt = ssum - mpar;
+      double err;
+      if (HELM) {
+        const double other = __shfl(dlt, (lane + NP) & 63, 64);
+        err = lane < NP ? sqrt(dlt * dlt + other * other) : 0.0;
+      } else {
+        err = lane < NP ? fabs(dlt) : 0.0;
+      }
+      err = wave_max(err);
+      if (err > eps && level + 1 <= NLMAX && sp + 4 <= STACK_CAP) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          double* ne = stack + (size_t)(sp + c) * ES;
+          if (lane == 0) {
+            ne[0] = cx[c]; ne[1] = cy[c]; ne[2] = cdx[c]; ne[3] = cdy[c]; ne[4] = (double)(level + 1);
+          }
+          if (lane < NC) ne[6 + lane] = mc[c];
+        }
+        sp += 4;
+      } else {
+        if (err > eps && level + 1 <= NLMAX && lane == 0) atomicAdd(overflow, 1);
+        result += ssum;
+      }
+      __syncthreads();
+    }
+    if (nref_out && lane == 0) nref_out[jt.x] = nref;
+    store_weights<NP, HELM>(buf, result, umat, wnear, iquad[jt.x] - 1);
+  }
+}
+
+// ------------------------------------------------------------------ 5. self
+template <int N, bool HELM>
+__global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__ list,
+                                             const int64_t* __restrict__ col_ind,
+                                             const int64_t* __restrict__ ixyzs,
+                                             const double* __restrict__ srccoefs, int64_t ndtarg,
+                                             const double* __restrict__ targs,
+                                             const double* __restrict__ uvs_targ,
+                                             const int64_t* __restrict__ iquad,
+                                             const double* __restrict__ umat, KPar kp, int nr, int nt,
+                                             double tau_panel, double* __restrict__ wnear) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  __shared__ double sc[9 * NP];
+  __shared__ double buf[WAVE * (NP + 2)];
+  const int lane = threadIdx.x;
+  const int idx = blockIdx.x;
+  if (idx >= nself) return;
+  const int2 jt = list[idx];
+  const int64_t ip = col_ind[jt.x] - 1;
+  const double* gsc = srccoefs + 9 * (ixyzs[ip] - 1);
+  for (int i = lane; i < 9 * NP; i += WAVE) sc[i] = gsc[i];
+  __syncthreads();
+  const double* tp = targs + ndtarg * (int64_t)jt.y;
+  const double tg[3] = {tp[0], tp[1], tp[2]};
+  const double u0 = uvs_targ[2 * (int64_t)jt.y], v0 = uvs_targ[2 * (int64_t)jt.y + 1];
+  double P[NP], g0[12];
+  koorn_eval<N>(u0, v0, P);
+  geom_eval<NP>(sc, P, g0);
+  const double g11 = g0[3] * g0[3] + g0[4] * g0[4] + g0[5] * g0[5];
+  const double g12 = g0[3] * g0[6] + g0[4] * g0[7] + g0[5] * g0[8];
+  const double g22 = g0[6] * g0[6] + g0[7] * g0[7] + g0[8] * g0[8];
+  const double l11 = sqrt(g11), l12 = g12 / l11, l22 = sqrt(g22 - l12 * l12);
+  const double detl = l11 * l22;
+  const double vx[3] = {0.0, 1.0, 0.0}, vy[3] = {0.0, 0.0, 1.0};
+  double acc = 0.0;
+  const int nn = nr * nt;
+  for (int e = 0; e < 3; e++) {
+    const double Ax = vx[e] - u0, Ay = vy[e] - v0;
+    const double Bx = vx[(e + 1) % 3] - u0, By = vy[(e + 1) % 3] - v0;
+    const double ax = l11 * Ax + l12 * Ay, ay = l22 * Ay;
+    const double bx = l11 * Bx + l12 * By, by = l22 * By;
+    const double len = sqrt((bx - ax) * (bx - ax) + (by - ay) * (by - ay));
+    const double ex = (bx - ax) / len, ey = (by - ay) / len;
+    const double ts = -(ax * ex + ay * ey);
+    const double fx = ax + ts * ex, fy = ay + ts * ey;
+    const double d = sqrt(fx * fx + fy * fy);
+    const double s0 = -ts, s1 = len - ts;
+    double segs[3];
+    int nseg = 0;
+    segs[nseg++] = s0;
+    if (s0 < 0.0 && s1 > 0.0) segs[nseg++] = 0.0;
+    segs[nseg++] = s1;
+    for (int sg = 0; sg + 1 < nseg; sg++) {
+      const double ta = asinh(segs[sg] / d), tb = asinh(segs[sg + 1] / d);
+      const double span = fabs(tb - ta) / tau_panel;
+      int npan = span < 64.0 ? (int)ceil(span) : 64;  // also maps NaN to 1 below
+      if (!(npan >= 1)) npan = 1;
+      for (int ipan = 0; ipan < npan; ipan++) {
+        const double t0 = ta + (tb - ta) * ipan / npan, t1 = ta + (tb - ta) * (ipan + 1) / npan;
+        for (int base = 0; base < nn; base += WAVE) {
+          const int k = base + lane;
+          if (k < nn) {
+            double* row = buf + lane * (NP + 2);
+            const int it = k / nr, ir = k % nr;
+            const double tau = t0 + (t1 - t0) * g_self_gl.x[it];
+            const double sh = sinh(tau), ch = cosh(tau);
+            const double wtau = g_self_gl.w[it] * (t1 - t0);
+            const double px = fx + d * sh * ex, py = fy + d * sh * ey;
+            const double rho = g_self_gl.x[ir];
+            const double xi0 = rho * px, xi1 = rho * py;
+            const double dv = xi1 / l22, du = (xi0 - l12 * dv) / l11;
+            double g[12];
+            koorn_geom_row<N>(u0 + du, v0 + dv, sc, g, row + 2);
+            const double jac = geom_finish(g);
+            const double w = g_self_gl.w[ir] * wtau * rho * d * d * ch / detl * jac;
+            double vr, vi;
+            kern_eval<HELM>(tg, g, kp, vr, vi);
+            row[0] = vr * w;
+            row[1] = vi * w;
+          }
+          const int nrows = nn - base < WAVE ? nn - base : WAVE;
+          acc += column_sum<NP, HELM>(buf, nrows);
+        }
+      }
+    }
+  }
+  store_weights<NP, HELM>(buf, acc, umat, wnear, iquad[jt.x] - 1);
+}
+
+template <typename T>
+T* dalloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+  return (T*)p;
+}
+
+#define HCHK(x)                                                                   \
+  do {                                                                            \
+    hipError_t e_ = (x);                                                          \
+    if (e_ != hipSuccess) {                                                       \
+      fprintf(stderr, "fmm3dbie_amd: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+      return -1;                                                                  \
+    }                                                                             \
+  } while (0)
+
+bool g_const_ready = false;
+
+int ensure_constants() {
+  if (g_const_ready) return 0;
+  KoornConst kc;
+  fmmhost::fill_koorn_const(&kc);
+  HCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_koorn), &kc, sizeof(kc)));
+  g_const_ready = true;
+  return 0;
+}
+
+int set_self_gl(int n) {
+  GLRule gl = {};
+  fmmhost::gauss_legendre01(n, gl.x, gl.w);
+  HCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_self_gl), &gl, sizeof(gl)));
+  return 0;
+}
+
+template <int N, bool HELM>
+int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  const int64_t npat = A.npatches;
+  // tables
+  int nqorder, nlev, nqorder_f;
+  double eps_adap;
+  fmmhost::quadparams_adap(A.eps, A.norder_branch, &nqorder, &eps_adap, &nlev, &nqorder_f);
+  std::vector<double> umat, rvuv(2 * NP), rvw(NP);
+  fmmhost::rv_umat(N, umat);
+  fmmhost::rv_nodes(N, rvuv.data(), rvw.data());
+  std::vector<double> xq, xw, fuv, fw;
+  const int kq = fmmhost::xg_rule(nqorder, xq, xw);
+  const int nf = fmmhost::xg_unif_nodes(nlev, nqorder_f, fuv, fw);
+  std::vector<double> xg(3 * kq), Pq((size_t)nf * NP), Bq((size_t)nf * NP), pols(NP);
+  for (int q = 0; q < kq; q++) { xg[3 * q] = xq[2 * q]; xg[3 * q + 1] = xq[2 * q + 1]; xg[3 * q + 2] = xw[q]; }
+  for (int q = 0; q < nf; q++) {
+    fmmhost::koorn_pols(fuv[2 * q], fuv[2 * q + 1], N, pols.data());
+    for (int l = 0; l < NP; l++) Pq[(size_t)q * NP + l] = pols[l];
+    for (int j = 0; j < NP; j++) {
+      double sacc = 0.0;
+      for (int l = 0; l < NP; l++) sacc += pols[l] * umat[l + (size_t)j * NP];
+      Bq[(size_t)q * NP + j] = fw[q] * sacc;
+    }
+  }
+  double *d_umat = dalloc<double>(NP * NP), *d_rvw = dalloc<double>(NP), *d_xg = dalloc<double>(3 * kq),
+         *d_fuv = dalloc<double>(2 * nf), *d_Pq = dalloc<double>((size_t)nf * NP),
+         *d_Bq = dalloc<double>((size_t)nf * NP);
+  double *d_cms = dalloc<double>(3 * npat), *d_rads = dalloc<double>(npat), *d_epsp = dalloc<double>(npat);
+  int *d_cnt = dalloc<int>(npat), *d_off = dalloc<int>(npat + 1), *d_fill = dalloc<int>(npat);
+  int *d_cls = dalloc<int>(A.nnz), *d_counts = dalloc<int>(4);
+  int2 *d_adap = dalloc<int2>(A.nnz), *d_self = dalloc<int2>(A.nnz), *d_fixed = dalloc<int2>(A.nnz);
+  int* d_nref = A.want_nref ? dalloc<int>(A.nnz) : nullptr;
+  if (!d_umat || !d_rvw || !d_xg || !d_fuv || !d_Pq || !d_Bq || !d_cms || !d_rads || !d_epsp || !d_cnt ||
+      !d_off || !d_fill || !d_cls || !d_counts || !d_adap || !d_self || !d_fixed) {
+    fprintf(stderr, "fmm3dbie_amd: device allocation failed in getnearquad\n");
+    return -2;
+  }
+  HCHK(hipMemcpyAsync(d_umat, umat.data(), sizeof(double) * NP * NP, hipMemcpyHostToDevice, s));
+  HCHK(hipMemcpyAsync(d_rvw, rvw.data(), sizeof(double) * NP, hipMemcpyHostToDevice, s));
+  HCHK(hipMemcpyAsync(d_xg, xg.data(), sizeof(double) * 3 * kq, hipMemcpyHostToDevice, s));
+  HCHK(hipMemcpyAsync(d_fuv, fuv.data(), sizeof(double) * 2 * nf, hipMemcpyHostToDevice, s));
+  HCHK(hipMemcpyAsync(d_Pq, Pq.data(), sizeof(double) * nf * NP, hipMemcpyHostToDevice, s));
+  HCHK(hipMemcpyAsync(d_Bq, Bq.data(), sizeof(double) * nf * NP, hipMemcpyHostToDevice, s));
+  HCHK(hipMemsetAsync(d_cnt, 0, sizeof(int) * npat, s));
+  HCHK(hipMemsetAsync(d_fill, 0, sizeof(int) * npat, s));
+  HCHK(hipMemsetAsync(d_counts, 0, sizeof(int) * 4, s));
+  KPar kp;
+  kp.zkr = A.zk[0]; kp.zki = A.zk[1];
+  kp.ar = A.alpha[0]; kp.ai = A.alpha[1];
+  kp.br = A.beta[0]; kp.bi = A.beta[1];
+
+  hipEvent_t ev[6];
+  for (auto& e : ev) HCHK(hipEventCreate(&e));
+  HCHK(hipEventRecord(ev[0], s));
+  const int tb = 256;
+  k_patch_prep<N><<<(npat + tb - 1) / tb, tb, 0, s>>>(npat, A.ixyzs, A.srccoefs, A.srcvals, d_rvw, eps_adap,
+                                                      d_cms, d_rads, d_epsp);
+  k_classify<<<(A.ntarg + tb - 1) / tb, tb, 0, s>>>(A.ntarg, A.ndtarg, A.targs, A.ipatch_id, A.row_ptr,
+                                                    A.col_ind, d_cms, d_rads, A.rfac0, d_cnt, d_adap,
+                                                    d_counts + 0, d_self, d_counts + 1, d_cls);
+  k_scan<<<1, 1024, 0, s>>>(npat, d_cnt, d_off);
+  k_scatter_fixed<<<(A.ntarg + tb - 1) / tb, tb, 0, s>>>(A.ntarg, A.row_ptr, A.col_ind, d_cls, d_off, d_fill,
+                                                         d_fixed);
+  int counts[4] = {0, 0, 0, 0};
+  HCHK(hipMemcpyAsync(counts, d_counts, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+  HCHK(hipStreamSynchronize(s));
+  const int nadap = counts[0], nself = counts[1];
+  HCHK(hipEventRecord(ev[1], s));
+  // fixed
+  const size_t lds_fixed = sizeof(double) * (9 * NP + 7 * (size_t)nf);
+  k_fixed<N, HELM><<<npat, 128, lds_fixed, s>>>(nf, d_fuv, d_Pq, d_Bq, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
+                                                A.iquad, d_off, d_fixed, kp, A.wnear);
+  HCHK(hipGetLastError());
+  HCHK(hipEventRecord(ev[2], s));
+  // adaptive
+  const int nwaves = A.adaptive_waves > 0 ? A.adaptive_waves : 4096;
+  constexpr int ES = 6 + 2 * NP;
+  double* d_stack = dalloc<double>((size_t)nwaves * STACK_CAP * ES);
+  if (!d_stack) return -2;
+  int* d_over = d_counts + 2;
+  int* d_ctr = d_counts + 3;
+  if (nadap > 0) {
+    const int g = nadap < nwaves ? nadap : nwaves;
+    k_adaptive<N, HELM><<<g, 64, 0, s>>>(nadap, d_adap, d_ctr, kq, d_xg, A.col_ind, A.ixyzs, A.srccoefs,
+                                         A.ndtarg, A.targs, A.iquad, d_epsp, d_umat, kp, d_stack, d_nref, d_over,
+                                         A.wnear);
+    HCHK(hipGetLastError());
+  }
+  HCHK(hipEventRecord(ev[3], s));
+  if (nself > 0) {
+    if (!A.uvs_targ) {
+      fprintf(stderr, "fmm3dbie_amd: self pairs present but no uvs_targ\n");
+      return -3;
+    }
+    k_self<N, HELM><<<nself, 64, 0, s>>>(nself, d_self, A.col_ind, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
+                                         A.uvs_targ, A.iquad, d_umat, kp, A.self_n, A.self_n, A.self_tau_panel,
+                                         A.wnear);
+    HCHK(hipGetLastError());
+  }
+  HCHK(hipEventRecord(ev[4], s));
+  HCHK(hipMemcpyAsync(counts, d_counts, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+  HCHK(hipStreamSynchronize(s));
+  if (st) {
+    float ms;
+    st->nself = nself;
+    st->nadap = nadap;
+    st->nfixed = (int)(A.nnz - nself - nadap);
+    st->overflow = counts[2];
+    st->nf = nf;
+    st->kq = kq;
+    hipEventElapsedTime(&ms, ev[0], ev[1]); st->ms_prep = ms;
+    hipEventElapsedTime(&ms, ev[1], ev[2]); st->ms_fixed = ms;
+    hipEventElapsedTime(&ms, ev[2], ev[3]); st->ms_adap = ms;
+    hipEventElapsedTime(&ms, ev[3], ev[4]); st->ms_self = ms;
+    st->nref_total = 0;
+    if (d_nref && A.nref_host) {
+      hipMemcpy(A.nref_host, d_nref, sizeof(int) * A.nnz, hipMemcpyDeviceToHost);
+    }
+    if (A.class_host) hipMemcpy(A.class_host, d_cls, sizeof(int) * A.nnz, hipMemcpyDeviceToHost);
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  void* frees[] = {d_umat, d_rvw, d_xg, d_fuv, d_Pq, d_Bq, d_cms, d_rads, d_epsp, d_cnt, d_off, d_fill,
+                   d_cls, d_counts, d_adap, d_self, d_fixed, d_nref, d_stack};
+  for (void* p : frees)
+    if (p) hipFree(p);
+  return 0;
+}
+
+template <bool HELM>
+int dispatch(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
+  switch (A.norder) {
+    case 1: return run_order<1, HELM>(A, st, s);
+    case 2: return run_order<2, HELM>(A, st, s);
+    case 3: return run_order<3, HELM>(A, st, s);
+    case 4: return run_order<4, HELM>(A, st, s);
+    case 5: return run_order<5, HELM>(A, st, s);
+    case 6: return run_order<6, HELM>(A, st, s);
+    case 7: return run_order<7, HELM>(A, st, s);
+    case 8: return run_order<8, HELM>(A, st, s);
+    case 9: return run_order<9, HELM>(A, st, s);
+    case 10: return run_order<10, HELM>(A, st, s);
+    default:
+      fprintf(stderr, "fmm3dbie_amd: patch order %d not supported (1..10)\n", A.norder);
+      return -4;
+  }
+}
+
+}  // namespace
+
+int fmm_nearquad_device(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
+  if (ensure_constants() != 0) return -1;
+  if (set_self_gl(A.self_n) != 0) return -1;
+  if (A.self_n < 1 || A.self_n > 16) {
+    fprintf(stderr, "fmm3dbie_amd: self rule size %d out of range (1..16)\n", A.self_n);
+    return -5;
+  }
+  return A.helm ? dispatch<true>(A, st, s) : dispatch<false>(A, st, s);
+}
+
+int fmm_koorn_constants() { return ensure_constants(); }

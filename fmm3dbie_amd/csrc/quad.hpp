@@ -1,0 +1,41 @@
+// Device-pointer interface of the near-quadrature pipeline (quad.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct NearQuadArgs {
+  bool helm;              // Helmholtz (complex weights) or Laplace (real weights)
+  int norder;             // patch order (uniform over patches)
+  int norder_branch;      // order used by the parameter table (average patch order)
+  int64_t npatches;
+  const int64_t* ixyzs;   // device, 1-based (npatches+1)
+  const double* srccoefs; // device (9, npts)
+  const double* srcvals;  // device (12, npts)
+  int64_t ndtarg, ntarg;
+  const double* targs;    // device (ndtarg, ntarg)
+  const int64_t* ipatch_id;  // device (ntarg) or null (all off-surface)
+  const double* uvs_targ;    // device (2, ntarg) or null
+  double eps;
+  double zk[2], alpha[2], beta[2];
+  int64_t nnz;
+  const int64_t* row_ptr;  // device, 1-based
+  const int64_t* col_ind;  // device, 1-based
+  const int64_t* iquad;    // device, 1-based
+  double rfac0;
+  double* wnear;           // device, nquad complex (interleaved) or real
+  int self_n;              // Gauss-Legendre points per direction of the self rule
+  double self_tau_panel;   // max sinh-parameter panel length of the self rule
+  int adaptive_waves;      // persistent wavefronts of the adaptive kernel (0 = default)
+  bool want_nref;
+  int* nref_host;          // optional: refinements per pair (nnz)
+  int* class_host;         // optional: pair class (nnz): 0 self, 1 adaptive, 2 fixed
+};
+
+struct NearQuadStats {
+  int nself, nadap, nfixed, overflow, nf, kq;
+  double ms_prep, ms_fixed, ms_adap, ms_self;
+  long long nref_total;
+};
+
+int fmm_nearquad_device(const NearQuadArgs& a, NearQuadStats* st, hipStream_t s);
+int fmm_koorn_constants();

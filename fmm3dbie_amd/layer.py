@@ -1,0 +1,363 @@
+"""Python mirror of the reference's layer-potential interface.
+
+Host drop-ins (numpy in, numpy out) call the reference-signature C-ABI symbols of
+libfmm3dbie_amd.so, which run on the GPU:
+
+* :func:`getnearquad_helm_comb_dir`, :func:`getnearquad_helm_comb_dir_eval`
+  (src/helm_wrappers/helm_comb_dir.f:107-434);
+* :func:`getnearquad_lap_comb_dir`, :func:`getnearquad_lap_comb_dir_eval`
+  (src/lap_wrappers/lap_comb_dir.f90:86-406);
+* :func:`lpcomp_helm_comb_dir_addsub`, :func:`lpcomp_lap_comb_dir_addsub`
+  (helm_comb_dir.f:658-850, lap_comb_dir.f90:633-805);
+* :func:`helm_comb_dir_solver_guru`, :func:`lap_comb_dir_solver_guru`.
+
+Device paths (torch tensors resident in HBM) are :class:`NearQuadDevice`,
+:class:`ApplyPlan`, :func:`gmres` and :func:`gmres_ir`.
+
+Errors: the reference prints and returns; here every entry point checks the library's
+status and raises ``RuntimeError``.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import lib
+from . import near
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _i8(v):
+    return C.byref(C.c_int64(int(v)))
+
+
+def _d(v):
+    return C.byref(C.c_double(float(v)))
+
+
+def _check(what):
+    rc = lib().fmm3dbie_last_error()
+    if rc != 0:
+        raise RuntimeError("%s failed (status %d)" % (what, rc))
+
+
+def _arr(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _z3(zpars):
+    return _arr(np.asarray(zpars, np.complex128), np.complex128).view(np.float64)
+
+
+# ------------------------------------------------------------------ host drop-ins
+
+def getnearquad_helm_comb_dir_eval(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, ndtarg, ntarg,
+                                   targs, ipatch_id, uvs_targ, eps, zpars, iquadtype, nnz, row_ptr, col_ind,
+                                   iquad, rfac0, nquad):
+    """wnear (nquad,) complex128 for u = alpha S_k + beta D_k, zpars = (k, alpha, beta)."""
+    wnear = np.zeros(int(nquad), np.complex128)
+    keep = [_arr(norders, np.int64), _arr(ixyzs, np.int64), _arr(iptype, np.int64), _arr(srccoefs, np.float64),
+            _arr(srcvals, np.float64), _arr(targs, np.float64), _arr(ipatch_id, np.int64),
+            _arr(uvs_targ, np.float64), _z3(zpars), _arr(row_ptr, np.int64), _arr(col_ind, np.int64),
+            _arr(iquad, np.int64)]
+    f = lib().getnearquad_helm_comb_dir_eval_
+    f(_i8(npatches), _p(keep[0]), _p(keep[1]), _p(keep[2]), _i8(npts), _p(keep[3]), _p(keep[4]), _i8(ndtarg),
+      _i8(ntarg), _p(keep[5]), _p(keep[6]), _p(keep[7]), _d(eps), _p(keep[8]), _i8(iquadtype), _i8(nnz),
+      _p(keep[9]), _p(keep[10]), _p(keep[11]), _d(rfac0), _i8(nquad), _p(wnear))
+    _check("getnearquad_helm_comb_dir_eval")
+    return wnear
+
+
+def getnearquad_helm_comb_dir(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, zpars, iquadtype,
+                              nnz, row_ptr, col_ind, iquad, rfac0, nquad):
+    wnear = np.zeros(int(nquad), np.complex128)
+    keep = [_arr(norders, np.int64), _arr(ixyzs, np.int64), _arr(iptype, np.int64), _arr(srccoefs, np.float64),
+            _arr(srcvals, np.float64), _z3(zpars), _arr(row_ptr, np.int64), _arr(col_ind, np.int64),
+            _arr(iquad, np.int64)]
+    lib().getnearquad_helm_comb_dir_(_i8(npatches), _p(keep[0]), _p(keep[1]), _p(keep[2]), _i8(npts),
+                                     _p(keep[3]), _p(keep[4]), _d(eps), _p(keep[5]), _i8(iquadtype), _i8(nnz),
+                                     _p(keep[6]), _p(keep[7]), _p(keep[8]), _d(rfac0), _i8(nquad), _p(wnear))
+    _check("getnearquad_helm_comb_dir")
+    return wnear
+
+
+def getnearquad_lap_comb_dir_eval(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, ndtarg, ntarg, targs,
+                                  ipatch_id, uvs_targ, eps, dpars, iquadtype, nnz, row_ptr, col_ind, iquad, rfac0,
+                                  nquad):
+    """wnear (nquad,) float64 for u = alpha S + beta D, dpars = (alpha, beta)."""
+    wnear = np.zeros(int(nquad), np.float64)
+    keep = [_arr(norders, np.int64), _arr(ixyzs, np.int64), _arr(iptype, np.int64), _arr(srccoefs, np.float64),
+            _arr(srcvals, np.float64), _arr(targs, np.float64), _arr(ipatch_id, np.int64),
+            _arr(uvs_targ, np.float64), _arr(dpars, np.float64), _arr(row_ptr, np.int64),
+            _arr(col_ind, np.int64), _arr(iquad, np.int64)]
+    lib().getnearquad_lap_comb_dir_eval_(_i8(npatches), _p(keep[0]), _p(keep[1]), _p(keep[2]), _i8(npts),
+                                         _p(keep[3]), _p(keep[4]), _i8(ndtarg), _i8(ntarg), _p(keep[5]),
+                                         _p(keep[6]), _p(keep[7]), _d(eps), _p(keep[8]), _i8(iquadtype), _i8(nnz),
+                                         _p(keep[9]), _p(keep[10]), _p(keep[11]), _d(rfac0), _i8(nquad), _p(wnear))
+    _check("getnearquad_lap_comb_dir_eval")
+    return wnear
+
+
+def getnearquad_lap_comb_dir(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, dpars, iquadtype, nnz,
+                             row_ptr, col_ind, iquad, rfac0, nquad):
+    wnear = np.zeros(int(nquad), np.float64)
+    keep = [_arr(norders, np.int64), _arr(ixyzs, np.int64), _arr(iptype, np.int64), _arr(srccoefs, np.float64),
+            _arr(srcvals, np.float64), _arr(dpars, np.float64), _arr(row_ptr, np.int64), _arr(col_ind, np.int64),
+            _arr(iquad, np.int64)]
+    lib().getnearquad_lap_comb_dir_(_i8(npatches), _p(keep[0]), _p(keep[1]), _p(keep[2]), _i8(npts), _p(keep[3]),
+                                    _p(keep[4]), _d(eps), _p(keep[5]), _i8(iquadtype), _i8(nnz), _p(keep[6]),
+                                    _p(keep[7]), _p(keep[8]), _d(rfac0), _i8(nquad), _p(wnear))
+    _check("getnearquad_lap_comb_dir")
+    return wnear
+
+
+def _lpcomp(sym, cplx, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, pars, nnz, row_ptr, col_ind,
+            iquad, nquad, wnear, novers, nptso, ixyzso, srcover, whtsover, sigma):
+    pot = np.zeros(int(npts), np.complex128 if cplx else np.float64)
+    keep = [_arr(norders, np.int64), _arr(ixyzs, np.int64), _arr(iptype, np.int64), _arr(srccoefs, np.float64),
+            _arr(srcvals, np.float64), _z3(pars) if cplx else _arr(pars, np.float64), _arr(row_ptr, np.int64),
+            _arr(col_ind, np.int64), _arr(iquad, np.int64),
+            _arr(wnear, np.complex128 if cplx else np.float64), _arr(novers, np.int64), _arr(ixyzso, np.int64),
+            _arr(srcover, np.float64), _arr(whtsover, np.float64),
+            _arr(sigma, np.complex128 if cplx else np.float64)]
+    dummy = np.zeros(4)
+    zero = _i8(0)
+    dp = keep[5] if not cplx else dummy
+    zp = keep[5] if cplx else dummy
+    getattr(lib(), sym)(_i8(npatches), _p(keep[0]), _p(keep[1]), _p(keep[2]), _i8(npts), _p(keep[3]), _p(keep[4]),
+                        _d(eps), _i8(0 if cplx else 2), _p(dp), _i8(3 if cplx else 0), _p(zp), zero, _p(dummy),
+                        _i8(nnz), _p(keep[6]), _p(keep[7]), _p(keep[8]), _i8(nquad), _i8(1), _p(keep[9]),
+                        _p(keep[10]), _i8(nptso), _p(keep[11]), _p(keep[12]), _p(keep[13]), _i8(0), _p(dummy),
+                        _i8(1), _p(keep[14]), _p(pot))
+    _check(sym)
+    return pot
+
+
+def lpcomp_helm_comb_dir_addsub(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, zpars, nnz,
+                                row_ptr, col_ind, iquad, nquad, wnear, novers, nptso, ixyzso, srcover, whtsover,
+                                sigma):
+    return _lpcomp("lpcomp_helm_comb_dir_addsub_", True, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals,
+                   eps, zpars, nnz, row_ptr, col_ind, iquad, nquad, wnear, novers, nptso, ixyzso, srcover,
+                   whtsover, sigma)
+
+
+def lpcomp_lap_comb_dir_addsub(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, dpars, nnz, row_ptr,
+                               col_ind, iquad, nquad, wnear, novers, nptso, ixyzso, srcover, whtsover, sigma):
+    return _lpcomp("lpcomp_lap_comb_dir_addsub_", False, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals,
+                   eps, dpars, nnz, row_ptr, col_ind, iquad, nquad, wnear, novers, nptso, ixyzso, srcover,
+                   whtsover, sigma)
+
+
+def _solver(sym, cplx, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, pars, numit, ifinout, rhs,
+            nnz, row_ptr, col_ind, iquad, nquad, wnear, novers, nptso, ixyzso, srcover, wover, eps_gmres):
+    soln = np.zeros(int(npts), np.complex128 if cplx else np.float64)
+    errs = np.zeros(int(numit) + 1)
+    niter = C.c_int64(0)
+    rres = C.c_double(0)
+    keep = [_arr(norders, np.int64), _arr(ixyzs, np.int64), _arr(iptype, np.int64), _arr(srccoefs, np.float64),
+            _arr(srcvals, np.float64), _z3(pars) if cplx else _arr(pars, np.float64),
+            _arr(rhs, np.complex128 if cplx else np.float64), _arr(row_ptr, np.int64), _arr(col_ind, np.int64),
+            _arr(iquad, np.int64), _arr(wnear, np.complex128 if cplx else np.float64), _arr(novers, np.int64),
+            _arr(ixyzso, np.int64), _arr(srcover, np.float64), _arr(wover, np.float64)]
+    getattr(lib(), sym)(_i8(npatches), _p(keep[0]), _p(keep[1]), _p(keep[2]), _i8(npts), _p(keep[3]), _p(keep[4]),
+                        _d(eps), _p(keep[5]), _i8(numit), _i8(ifinout), _p(keep[6]), _i8(nnz), _p(keep[7]),
+                        _p(keep[8]), _p(keep[9]), _i8(nquad), _i8(1), _p(keep[10]), _p(keep[11]), _i8(nptso),
+                        _p(keep[12]), _p(keep[13]), _p(keep[14]), _d(eps_gmres), C.byref(niter), _p(errs),
+                        C.byref(rres), _p(soln))
+    _check(sym)
+    return niter.value, errs[: niter.value], rres.value, soln
+
+
+def helm_comb_dir_solver_guru(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, zpars, numit, ifinout,
+                              rhs, nnz, row_ptr, col_ind, iquad, nquad, wnear, novers, nptso, ixyzso, srcover, wover,
+                              eps_gmres):
+    """(niter, errs, rres, soln) -- exterior problems use ifinout=1 (zid = beta/2)."""
+    return _solver("helm_comb_dir_solver_guru_", True, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals,
+                   eps, zpars, numit, ifinout, rhs, nnz, row_ptr, col_ind, iquad, nquad, wnear, novers, nptso,
+                   ixyzso, srcover, wover, eps_gmres)
+
+
+def lap_comb_dir_solver_guru(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, dpars, numit, ifinout,
+                             rhs, nnz, row_ptr, col_ind, iquad, nquad, wnear, novers, nptso, ixyzso, srcover, wover,
+                             eps_gmres):
+    return _solver("lap_comb_dir_solver_guru_", False, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals,
+                   eps, dpars, numit, ifinout, rhs, nnz, row_ptr, col_ind, iquad, nquad, wnear, novers, nptso,
+                   ixyzso, srcover, wover, eps_gmres)
+
+
+# ------------------------------------------------------------------ device paths
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def kpar6(helm, pars):
+    if helm:
+        z = np.asarray(pars, np.complex128)
+        return np.array([z[0].real, z[0].imag, z[1].real, z[1].imag, z[2].real, z[2].imag])
+    a, b = pars
+    return np.array([0.0, 0.0, float(a), 0.0, float(b), 0.0])
+
+
+class NearQuadDevice:
+    """Near quadrature with device-resident inputs (the benchmarked path).
+
+    geom: surface dict; targets default to the on-surface nodes (getnearquad_*_comb_dir).
+    """
+
+    def __init__(self, geom, helm, pars, eps, row_ptr, col_ind, iquad, rfac0=1.25, device="cuda", targets=None,
+                 ipatch_id=None, uvs_targ=None):
+        torch = _torch()
+        self.helm = bool(helm)
+        self.kp = kpar6(helm, pars)
+        self.eps = float(eps)
+        self.rfac0 = float(rfac0)
+        norders = np.asarray(geom["norders"], np.int64)
+        if not np.all(norders == norders[0]):
+            raise ValueError("patches of one order only")
+        self.norder = int(norders[0])
+        self.norder_branch = near.norder_avg(geom)
+        self.npatches = len(norders)
+        if targets is None:
+            targets = geom["srcvals"]
+            ipatch_id, uvs_targ = near.get_patch_id_uvs(geom)
+        self.ntarg, self.ndtarg = targets.shape
+        t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a, dtype=dt), device=device)  # noqa: E731
+        self.d_ixyzs = t(geom["ixyzs"], np.int64)
+        self.d_srccoefs = t(geom["srccoefs"], np.float64)
+        self.d_srcvals = t(geom["srcvals"], np.float64)
+        self.d_targs = t(targets, np.float64)
+        self.d_pid = t(ipatch_id, np.int64) if ipatch_id is not None else None
+        self.d_uvs = t(uvs_targ, np.float64) if uvs_targ is not None else None
+        self.d_row = t(row_ptr, np.int64)
+        self.d_col = t(col_ind, np.int64)
+        self.d_iquad = t(iquad, np.int64)
+        self.nnz = len(col_ind)
+        self.nquad = int(iquad[-1]) - 1
+        self.wnear = torch.zeros(self.nquad, dtype=torch.complex128 if self.helm else torch.float64, device=device)
+        self.stats = None
+
+    def run(self, stream=None, want_detail=False):
+        torch = _torch()
+        st = np.zeros(12)
+        nref = np.zeros(self.nnz, np.int32) if want_detail else None
+        cls = np.zeros(self.nnz, np.int32) if want_detail else None
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().fmm3dbie_nearquad_dev(
+            int(self.helm), self.norder, self.norder_branch, self.npatches, _dptr(self.d_ixyzs),
+            _dptr(self.d_srccoefs), _dptr(self.d_srcvals), self.ndtarg, self.ntarg, _dptr(self.d_targs),
+            _dptr(self.d_pid) if self.d_pid is not None else None,
+            _dptr(self.d_uvs) if self.d_uvs is not None else None, self.eps, _p(self.kp), self.nnz,
+            _dptr(self.d_row), _dptr(self.d_col), _dptr(self.d_iquad), self.rfac0, _dptr(self.wnear), 10, 1.0,
+            C.c_void_p(s.cuda_stream), _p(st), _p(nref) if want_detail else None, _p(cls) if want_detail else None)
+        if rc != 0:
+            raise RuntimeError("fmm3dbie_nearquad_dev failed (status %d)" % rc)
+        self.stats = dict(nself=int(st[0]), nadap=int(st[1]), nfixed=int(st[2]), overflow=int(st[3]), nf=int(st[4]),
+                          kq=int(st[5]), ms_prep=st[6], ms_fixed=st[7], ms_adap=st[8], ms_self=st[9])
+        if want_detail:
+            self.stats["nref"] = nref
+            self.stats["cls"] = cls
+        return self.wnear
+
+
+class ApplyPlan:
+    """Device-resident apply (lpcomp_*_addsub) reused across matvecs and RHS."""
+
+    def __init__(self, geom, helm, pars, row_ptr, col_ind, iquad, wnear, novers, ixyzso, srcover, whtsover,
+                 stream=None):
+        torch = _torch()
+        self.helm = bool(helm)
+        self.kp = kpar6(helm, pars)
+        self.npts = int(geom["ixyzs"][-1]) - 1
+        keep = [_arr(geom["norders"], np.int64), _arr(geom["ixyzs"], np.int64), _arr(geom["srcvals"], np.float64),
+                _arr(row_ptr, np.int64), _arr(col_ind, np.int64), _arr(iquad, np.int64), _arr(novers, np.int64),
+                _arr(ixyzso, np.int64), _arr(srcover, np.float64), _arr(whtsover, np.float64)]
+        if isinstance(wnear, np.ndarray):
+            wk = _arr(wnear, np.complex128 if self.helm else np.float64)
+            wptr, ondev = _p(wk), 0
+        else:
+            wk = wnear.contiguous()
+            wptr, ondev = _dptr(wk), 1
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self.handle = lib().fmm3dbie_apply_plan_create(
+            int(self.helm), len(keep[0]), _p(keep[0]), _p(keep[1]), self.npts, _p(keep[2]), _p(self.kp),
+            len(keep[4]), _p(keep[3]), _p(keep[4]), _p(keep[5]), int(iquad[-1]) - 1, wptr, ondev, _p(keep[6]),
+            len(whtsover), _p(keep[7]), _p(keep[8]), _p(keep[9]), C.c_void_p(s.cuda_stream))
+        if not self.handle:
+            raise RuntimeError("apply plan creation failed (status %d)" % lib().fmm3dbie_last_error())
+
+    def apply(self, sigma, prec=64, mode=0, stream=None, times=None):
+        """sigma: complex tensor (npts,) or (nrhs, npts); returns pot of the same shape."""
+        torch = _torch()
+        sig = sigma.contiguous()
+        nrhs = 1 if sig.dim() == 1 else sig.shape[0]
+        pot = torch.empty_like(sig)
+        tm = np.zeros(4)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().fmm3dbie_apply_plan_apply(self.handle, prec, mode, _dptr(sig), _dptr(pot), nrhs,
+                                             C.c_void_p(s.cuda_stream), _p(tm) if times is not None else None)
+        if rc != 0:
+            raise RuntimeError("apply failed (status %d)" % rc)
+        if times is not None:
+            times[:] = tm
+        return pot
+
+    def close(self):
+        h = getattr(self, "handle", None)
+        if h:
+            lib().fmm3dbie_apply_plan_destroy(h)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
+
+
+def gmres(plan, rhs, zid, wts, numit=200, eps_gmres=1e-7, prec=64, mode=0, stream=None):
+    """GMRES for (zid I + K) x = rhs on the device.  rhs (npts,) or (nrhs, npts) complex.
+    Returns (x, niter[nrhs], errs[nrhs, numit], rres[nrhs], applies)."""
+    torch = _torch()
+    r = rhs.contiguous()
+    nrhs = 1 if r.dim() == 1 else r.shape[0]
+    x = torch.empty_like(r)
+    niter = np.zeros(nrhs, np.int32)
+    errs = np.zeros((nrhs, numit))
+    rres = np.zeros(nrhs)
+    st = np.zeros(2, np.int32)
+    z = np.array([complex(zid).real, complex(zid).imag])
+    s = stream if stream is not None else torch.cuda.current_stream()
+    rc = lib().fmm3dbie_gmres_dev(plan.handle, prec, mode, _p(z), _dptr(r), _dptr(x), _dptr(wts), nrhs, numit,
+                                  eps_gmres, _p(niter), _p(errs), _p(rres), C.c_void_p(s.cuda_stream), _p(st))
+    if rc != 0:
+        raise RuntimeError("gmres failed (status %d)" % rc)
+    return x, niter, errs, rres, int(st[0])
+
+
+def gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=1e-4, tol=1e-6, max_outer=10, mode=0, stream=None):
+    """Mixed precision: FP32 inner GMRES + FP64 residual refinement.
+    Returns (x, outer, inner_total, rres[nrhs], fp32 applies, fp64 applies)."""
+    torch = _torch()
+    r = rhs.contiguous()
+    nrhs = 1 if r.dim() == 1 else r.shape[0]
+    x = torch.empty_like(r)
+    outer = C.c_int(0)
+    inner = C.c_int(0)
+    rres = np.zeros(nrhs)
+    st = np.zeros(2, np.int32)
+    z = np.array([complex(zid).real, complex(zid).imag])
+    s = stream if stream is not None else torch.cuda.current_stream()
+    rc = lib().fmm3dbie_gmres_ir_dev(plan.handle, mode, _p(z), _dptr(r), _dptr(x), _dptr(wts), nrhs, numit,
+                                     eps_inner, tol, max_outer, C.byref(outer), C.byref(inner), _p(rres),
+                                     C.c_void_p(s.cuda_stream), _p(st))
+    if rc != 0:
+        raise RuntimeError("gmres_ir failed (status %d)" % rc)
+    return x, outer.value, inner.value, rres, int(st[0]), int(st[1])

@@ -1,0 +1,51 @@
+"""Set-up of a combined-field Dirichlet problem the way the reference's solver does it
+(helm_comb_dir_solver, helm_comb_dir.f:1808-2065; lap_comb_dir_solver,
+lap_comb_dir.f90:1189-1458): radii -> near lists -> quadrature offsets -> far order ->
+oversampled geometry -> smooth weights.  Host-side numpy; the quadrature itself and the
+apply run on the GPU through :mod:`fmm3dbie_amd.layer`."""
+import numpy as np
+
+from . import near
+
+
+def setup(geom, eps, helm, pars, far_order=True, nover_fixed=None):
+    """Returns a dict with row_ptr, col_ind, iquad (1-based), rfac, rfac0, cms, rads,
+    novers, ixyzso, srcover, whtsover, wts, ipatch_id, uvs_targ, nnz, nquad."""
+    norder = near.norder_avg(geom)
+    rfac, rfac0 = near.get_rfacs(norder)
+    cms, rads = near.get_centroid_rads(geom)
+    targs = geom["srcvals"][:, :3]
+    row_ptr, col_ind = near.findnear(cms, rads * rfac, targs)
+    iquad = near.get_iquad_rsc(geom, row_ptr, col_ind)
+    if helm:
+        zk = complex(np.asarray(pars, np.complex128)[0])
+        beta = complex(np.asarray(pars, np.complex128)[2])
+    else:
+        zk = 0.0
+        beta = float(pars[1])
+    ikerorder = 0 if abs(beta) > 1e-16 else -1
+    if nover_fixed is not None:
+        novers = np.full(len(geom["norders"]), int(nover_fixed), np.int64)
+    elif far_order:
+        novers, _ = near.get_far_order(eps, geom, cms, rads, targs, ikerorder, zk.real if helm else 0.0, row_ptr,
+                                       col_ind, rfac)
+    else:
+        novers = np.asarray(geom["norders"], np.int64).copy()
+    srcover, whtsover, ixyzso = near.oversample_geom(geom, novers)
+    pid, uvs = near.get_patch_id_uvs(geom)
+    return dict(row_ptr=row_ptr, col_ind=col_ind, iquad=iquad, rfac=rfac, rfac0=rfac0, cms=cms, rads=rads,
+                novers=novers, ixyzso=ixyzso, srcover=srcover, whtsover=whtsover, wts=near.get_qwts(geom),
+                ipatch_id=pid, uvs_targ=uvs, nnz=len(col_ind), nquad=int(iquad[-1]) - 1)
+
+
+def point_source(xyz, x0, zk=None):
+    """e^{ik|x-x0|}/(4 pi |x-x0|) (Helmholtz) or 1/(4 pi |x-x0|) (Laplace)."""
+    r = np.linalg.norm(xyz - np.asarray(x0)[None, :], axis=1)
+    if zk is None:
+        return 1.0 / (4 * np.pi * r)
+    return np.exp(1j * zk * r) / (4 * np.pi * r)
+
+
+def plane_wave_dirs(n, seed):
+    g = np.random.default_rng(seed).standard_normal((n, 3))
+    return g / np.linalg.norm(g, axis=1, keepdims=True)
