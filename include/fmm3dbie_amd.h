@@ -1,0 +1,193 @@
+/*
+ * fmm3dbie_amd: MI355X (gfx950) implementation of fmm3dbie's locally corrected
+ * layer-potential hot path, exported as a C ABI.
+ *
+ * Reference-signature entry points keep the Fortran calling convention of the
+ * reference (gfortran mangling: lower case + trailing underscore; every argument by
+ * pointer; integer*8 -> int64_t; real*8 -> double; complex*16 -> two doubles (re, im);
+ * arrays column-major; index arrays 1-based).  They take host arrays, run on the GPU
+ * and return host arrays; the caller owns every buffer.  The reference has no error
+ * return; here a nonzero status of the last call is available from
+ * fmm3dbie_last_error().
+ *
+ * Supported: triangular RV patches (iptype = 1) of one order in 1..10, iquadtype = 1,
+ * nker = 1, ndim = 1.
+ */
+#ifndef FMM3DBIE_AMD_H
+#define FMM3DBIE_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- near quadrature --------------------------------------------------------------- */
+
+/* src/helm_wrappers/helm_comb_dir.f:107-244  getnearquad_helm_comb_dir
+ * zpars(3) = (k, alpha, beta); wnear(nquad) complex. */
+void getnearquad_helm_comb_dir_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs,
+                                const int64_t *iptype, const int64_t *npts, const double *srccoefs,
+                                const double *srcvals, const double *eps, const double *zpars,
+                                const int64_t *iquadtype, const int64_t *nnz, const int64_t *row_ptr,
+                                const int64_t *col_ind, const int64_t *iquad, const double *rfac0,
+                                const int64_t *nquad, double *wnear);
+
+/* src/helm_wrappers/helm_comb_dir.f:250-434  getnearquad_helm_comb_dir_eval */
+void getnearquad_helm_comb_dir_eval_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs,
+                                     const int64_t *iptype, const int64_t *npts, const double *srccoefs,
+                                     const double *srcvals, const int64_t *ndtarg, const int64_t *ntarg,
+                                     const double *targs, const int64_t *ipatch_id, const double *uvs_targ,
+                                     const double *eps, const double *zpars, const int64_t *iquadtype,
+                                     const int64_t *nnz, const int64_t *row_ptr, const int64_t *col_ind,
+                                     const int64_t *iquad, const double *rfac0, const int64_t *nquad,
+                                     double *wnear);
+
+/* src/lap_wrappers/lap_comb_dir.f90:86-227  getnearquad_lap_comb_dir
+ * dpars(2) = (alpha, beta); wnear(nquad) real. */
+void getnearquad_lap_comb_dir_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs,
+                               const int64_t *iptype, const int64_t *npts, const double *srccoefs,
+                               const double *srcvals, const double *eps, const double *dpars,
+                               const int64_t *iquadtype, const int64_t *nnz, const int64_t *row_ptr,
+                               const int64_t *col_ind, const int64_t *iquad, const double *rfac0,
+                               const int64_t *nquad, double *wnear);
+
+/* src/lap_wrappers/lap_comb_dir.f90:228-406  getnearquad_lap_comb_dir_eval */
+void getnearquad_lap_comb_dir_eval_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs,
+                                    const int64_t *iptype, const int64_t *npts, const double *srccoefs,
+                                    const double *srcvals, const int64_t *ndtarg, const int64_t *ntarg,
+                                    const double *targs, const int64_t *ipatch_id, const double *uvs_targ,
+                                    const double *eps, const double *dpars, const int64_t *iquadtype,
+                                    const int64_t *nnz, const int64_t *row_ptr, const int64_t *col_ind,
+                                    const int64_t *iquad, const double *rfac0, const int64_t *nquad,
+                                    double *wnear);
+
+/* ---- apply (the zgmres_guru/dgmres_guru fker signature) ----------------------------- */
+
+/* src/helm_wrappers/helm_comb_dir.f:658-850  lpcomp_helm_comb_dir_addsub
+ * sigma(npts), pot(npts) complex; FMM3D replaced by an exact tiled kernel sum. */
+void lpcomp_helm_comb_dir_addsub_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs,
+                                  const int64_t *iptype, const int64_t *npts, const double *srccoefs,
+                                  const double *srcvals, const double *eps, const int64_t *ndd,
+                                  const double *dpars, const int64_t *ndz, const double *zpars,
+                                  const int64_t *ndi, const int64_t *ipars, const int64_t *nnz,
+                                  const int64_t *row_ptr, const int64_t *col_ind, const int64_t *iquad,
+                                  const int64_t *nquad, const int64_t *nker, const double *wnear,
+                                  const int64_t *novers, const int64_t *nptso, const int64_t *ixyzso,
+                                  const double *srcover, const double *whtsover, const int64_t *lwork,
+                                  const double *work, const int64_t *ndim, const double *sigma, double *pot);
+
+/* src/lap_wrappers/lap_comb_dir.f90:633-805  lpcomp_lap_comb_dir_addsub (dpars = alpha, beta) */
+void lpcomp_lap_comb_dir_addsub_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs,
+                                 const int64_t *iptype, const int64_t *npts, const double *srccoefs,
+                                 const double *srcvals, const double *eps, const int64_t *ndd,
+                                 const double *dpars, const int64_t *ndz, const double *zpars,
+                                 const int64_t *ndi, const int64_t *ipars, const int64_t *nnz,
+                                 const int64_t *row_ptr, const int64_t *col_ind, const int64_t *iquad,
+                                 const int64_t *nquad, const int64_t *nker, const double *wnear,
+                                 const int64_t *novers, const int64_t *nptso, const int64_t *ixyzso,
+                                 const double *srcover, const double *whtsover, const int64_t *lwork,
+                                 const double *work, const int64_t *ndim, const double *sigma, double *pot);
+
+/* ---- solvers ------------------------------------------------------------------------ */
+
+/* src/helm_wrappers/helm_comb_dir.f:2071-2229  helm_comb_dir_solver_guru (GMRES on the GPU) */
+void helm_comb_dir_solver_guru_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs,
+                                const int64_t *iptype, const int64_t *npts, const double *srccoefs,
+                                const double *srcvals, const double *eps, const double *zpars,
+                                const int64_t *numit, const int64_t *ifinout, const double *rhs,
+                                const int64_t *nnz, const int64_t *row_ptr, const int64_t *col_ind,
+                                const int64_t *iquad, const int64_t *nquad, const int64_t *nker,
+                                const double *wnear, const int64_t *novers, const int64_t *npts_over,
+                                const int64_t *ixyzso, const double *srcover, const double *wover,
+                                const double *eps_gmres, int64_t *niter, double *errs, double *rres,
+                                double *soln);
+
+/* src/lap_wrappers/lap_comb_dir.f90:1459-1650  lap_comb_dir_solver_guru */
+void lap_comb_dir_solver_guru_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs,
+                               const int64_t *iptype, const int64_t *npts, const double *srccoefs,
+                               const double *srcvals, const double *eps, const double *dpars,
+                               const int64_t *numit, const int64_t *ifinout, const double *rhs,
+                               const int64_t *nnz, const int64_t *row_ptr, const int64_t *col_ind,
+                               const int64_t *iquad, const int64_t *nquad, const int64_t *nker,
+                               const double *wnear, const int64_t *novers, const int64_t *nptso,
+                               const int64_t *ixyzso, const double *srcover, const double *whtsover,
+                               const double *eps_gmres, int64_t *niter, double *errs, double *rres,
+                               double *soln);
+
+/* ---- bookkeeping utilities ---------------------------------------------------------- */
+
+/* src/quadratures/near_field_routs.f:826-873  get_rfacs */
+void get_rfacs_(const int64_t *norder, const int64_t *iptype, double *rfac, double *rfac0);
+/* src/quadratures/near_field_routs.f:765-820  get_iquad_rsc */
+void get_iquad_rsc_(const int64_t *npatches, const int64_t *ixyzs, const int64_t *npts, const int64_t *nnz,
+                    const int64_t *row_ptr, const int64_t *col_ind, int64_t *iquad);
+/* src/quadratures/near_field_routs.f:998-1057  get_near_corr_max */
+void get_near_corr_max_(const int64_t *ntarg, const int64_t *row_ptr, const int64_t *nnz,
+                        const int64_t *col_ind, const int64_t *npatches, const int64_t *ixyzs, int64_t *nmax);
+/* src/common/get_fmm_thresh.f:1-90  get_fmm_thresh */
+void get_fmm_thresh_(const int64_t *nds, const int64_t *ns, const double *src, const int64_t *ndt,
+                     const int64_t *nt, const double *trg, double *thresh);
+/* src/common/sparse_reps.f:38-72  rsc_to_csc */
+void rsc_to_csc_(const int64_t *ncol, const int64_t *nrow, const int64_t *nnz, const int64_t *row_ptr,
+                 const int64_t *col_ind, int64_t *col_ptr, int64_t *row_ind, int64_t *iper);
+
+/* ---- discretization tables (host; no GPU needed) ------------------------------------- */
+
+/* src/tria_routs/koornexps.f90:57-154  get_vioreanu_nodes/get_vioreanu_wts: returns p */
+int fmm3dbie_rv_nodes(int norder, double *uvs, double *wts);
+/* src/tria_routs/koornexps.f90:568-601  koorn_vals2coefs: umat (p x p, column-major) */
+int fmm3dbie_rv_umat(int norder, double *umat);
+/* src/tria_routs/koornexps.f90:909-1016  koornf_pols */
+void fmm3dbie_koorn_pols(double u, double v, int nmax, double *pols);
+/* src/tria_routs/triasymq.f:89-158  triasymq on the simplex: returns the node count */
+int fmm3dbie_xg_rule(int deg, double *uvs, double *wts);
+
+/* ---- device-pointer interface (inputs resident in HBM, caller's stream) -------------- */
+/* No single reference counterpart: the same computations as the entries above with
+ * device pointers, for callers that keep the discretization on the GPU. */
+
+/* Status of the last call (0 = success). */
+int fmm3dbie_last_error(void);
+
+/* getnearquad_*_comb_dir(_eval) on device buffers.  kpar6 = (k re, k im, alpha re,
+ * alpha im, beta re, beta im); helm = 0 selects the Laplace kernel (real weights).
+ * stats (optional, 12 doubles): nself, nadap, nfixed, overflow, nf, kq, and the
+ * prep/fixed/adaptive/self kernel times in ms.  nref_host/class_host (optional, nnz
+ * ints): refinements per pair and pair class (0 self, 1 adaptive, 2 fixed). */
+int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatches, const int64_t *ixyzs,
+                          const double *srccoefs, const double *srcvals, int64_t ndtarg, int64_t ntarg,
+                          const double *targs, const int64_t *ipatch_id, const double *uvs_targ, double eps,
+                          const double *kpar6, int64_t nnz, const int64_t *row_ptr, const int64_t *col_ind,
+                          const int64_t *iquad, double rfac0, double *wnear, int self_n, double self_tau_panel,
+                          void *stream, double *stats, int *nref_host, int *class_host);
+
+/* Device-resident apply plan for lpcomp_*_addsub (host bookkeeping arrays; wnear on the
+ * host or, with wnear_on_device = 1, already in HBM). */
+void *fmm3dbie_apply_plan_create(int helm, int64_t npatches, const int64_t *norders, const int64_t *ixyzs,
+                                 int64_t npts, const double *srcvals, const double *kpar6, int64_t nnz,
+                                 const int64_t *row_ptr, const int64_t *col_ind, const int64_t *iquad,
+                                 int64_t nquad, const double *wnear, int wnear_on_device, const int64_t *novers,
+                                 int64_t nptso, const int64_t *ixyzso, const double *srcover,
+                                 const double *whtsover, void *stream);
+/* prec 64: complex128 sigma/pot; prec 32: complex64 (near patches masked in the tiled
+ * sum).  mode 0: add/subtract as the reference; 1: masked.  nrhs <= 64 columns of
+ * length npts.  times (optional, 4 doubles): oversample, far, csr, subtract ms. */
+int fmm3dbie_apply_plan_apply(void *plan, int prec, int mode, const void *sigma, void *pot, int nrhs,
+                              void *stream, double *times);
+int fmm3dbie_apply_plan_info(void *plan, int64_t *out4);
+void fmm3dbie_apply_plan_destroy(void *plan);
+
+/* src/common/gmres_routs.f90:13-371  zgmres_guru on a device plan (nrhs systems in
+ * lockstep, one batched apply per iteration). */
+int fmm3dbie_gmres_dev(void *plan, int prec, int mode, const double *zid, const void *rhs, void *soln,
+                       const double *wts, int nrhs, int numit, double eps_gmres, int *niter, double *errs,
+                       double *rres, void *stream, int *stats);
+/* No reference counterpart: FP32/complex64 inner GMRES with FP64 residual refinement. */
+int fmm3dbie_gmres_ir_dev(void *plan, int mode, const double *zid, const void *rhs, void *soln,
+                          const double *wts, int nrhs, int numit, double eps_inner, double tol, int max_outer,
+                          int *outer, int *inner_total, double *rres, void *stream, int *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

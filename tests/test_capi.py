@@ -1,0 +1,73 @@
+"""The C-ABI library loads and exports every symbol include/fmm3dbie_amd.h declares;
+host-only entry points (tables, bookkeeping) are checked without a GPU."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared():
+    txt = (ROOT / "include" / "fmm3dbie_amd.h").read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b([a-z_0-9]+)\s*\(", txt)) - {"if", "defined"})
+
+
+def test_library_exports_header_symbols():
+    from fmm3dbie_amd import lib
+    L = lib()
+    names = _declared()
+    assert "getnearquad_helm_comb_dir_" in names and "lpcomp_lap_comb_dir_addsub_" in names
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_host_tables_match_oracle():
+    from fmm3dbie_amd import basis
+    from oracle import oracle as O
+    for n in [0, 3, 4, 6, 10, 20]:
+        u1, w1 = basis.rv_nodes(n)
+        u2, w2 = O.rv_nodes(n)
+        assert np.array_equal(u1, u2) and np.array_equal(w1, w2)
+        if n <= 10:
+            assert np.abs(basis.rv_umat(n) - O.rv_umat(n)).max() < 1e-9
+            assert np.abs(basis.koorn_pols(0.2, 0.3, n) - O.koorn_pols(0.2, 0.3, n)).max() < 1e-13
+    for d in [12, 16, 22]:
+        a, b = basis.xg_rule(d)
+        c, e = O.xg_rule(d)
+        assert np.array_equal(a, c) and np.array_equal(b, e)
+
+
+def test_bookkeeping_entry_points():
+    from fmm3dbie_amd import lib
+    L = lib()
+    i8 = lambda v: C.byref(C.c_int64(v))  # noqa: E731
+    rf, rf0 = C.c_double(), C.c_double()
+    for n, want in [(2, 2.75), (4, 2.0), (6, 2.0), (8, 1.25)]:
+        L.get_rfacs_(i8(n), i8(1), C.byref(rf), C.byref(rf0))
+        assert rf.value == want and rf0.value == 1.25
+    ixyzs = np.array([1, 11, 21, 36], np.int64)
+    row_ptr = np.array([1, 3, 4], np.int64)
+    col_ind = np.array([1, 3, 2], np.int64)
+    iquad = np.zeros(4, np.int64)
+    L.get_iquad_rsc_(i8(3), C.c_void_p(ixyzs.ctypes.data), i8(2), i8(3), C.c_void_p(row_ptr.ctypes.data),
+                     C.c_void_p(col_ind.ctypes.data), C.c_void_p(iquad.ctypes.data))
+    assert iquad.tolist() == [1, 11, 26, 36]
+    nmax = C.c_int64()
+    L.get_near_corr_max_(i8(2), C.c_void_p(row_ptr.ctypes.data), i8(3), C.c_void_p(col_ind.ctypes.data), i8(3),
+                         C.c_void_p(ixyzs.ctypes.data), C.byref(nmax))
+    assert nmax.value == 25
+    col_ptr = np.zeros(4, np.int64)
+    row_ind = np.zeros(3, np.int64)
+    iper = np.zeros(3, np.int64)
+    L.rsc_to_csc_(i8(3), i8(2), i8(3), C.c_void_p(row_ptr.ctypes.data), C.c_void_p(col_ind.ctypes.data),
+                  C.c_void_p(col_ptr.ctypes.data), C.c_void_p(row_ind.ctypes.data), C.c_void_p(iper.ctypes.data))
+    assert col_ptr.tolist() == [1, 2, 3, 4] and row_ind.tolist() == [1, 2, 1] and iper.tolist() == [1, 3, 2]
+    src = np.array([[0, 0, 0], [1, 2, 0.5]], np.float64)
+    trg = np.array([[-1, 0, 3.0]], np.float64)
+    th = C.c_double()
+    L.get_fmm_thresh_(i8(3), i8(2), C.c_void_p(src.ctypes.data), i8(3), i8(1), C.c_void_p(trg.ctypes.data),
+                      C.byref(th))
+    assert th.value == 2.0 ** -51 * 3.0

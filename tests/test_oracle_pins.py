@@ -1,0 +1,284 @@
+"""Pins of the CPU oracle (and of the shared rule tables) against the reference's own
+golden data, closed forms and independent computations.  CPU only."""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def test_xg_rules_exact_and_positive():
+    # triasymq degree-d rules integrate every monomial of degree <= d on the simplex
+    for d in range(1, 31):
+        uv, w = O.xg_rule(d)
+        assert np.all(w > 0)
+        assert np.all(uv >= -1e-15) and np.all(uv.sum(1) <= 1 + 1e-15)
+        assert abs(w.sum() - 0.5) < 1e-14
+        for i in range(d + 1):
+            for j in range(d + 1 - i):
+                ex = math.factorial(i) * math.factorial(j) / math.factorial(i + j + 2)
+                assert abs((w * uv[:, 0] ** i * uv[:, 1] ** j).sum() - ex) <= 1e-13 * ex
+
+
+def test_rv_tables_counts_weights():
+    for n in range(21):
+        uv, w = O.rv_nodes(n)
+        assert len(w) == (n + 1) * (n + 2) // 2
+        assert abs(w.sum() - 0.5) < 1e-14
+        assert np.all(w > 0)
+
+
+def test_koornwinder_orthonormal_and_umat():
+    for n in [1, 3, 4, 6, 8]:
+        uv, w = O.xg_rule(2 * n)
+        P = np.stack([O.koorn_pols(u, v, n) for (u, v) in uv])
+        G = (P * w[:, None]).T @ P
+        assert np.abs(G - np.eye(len(G))).max() < 1e-12
+        rv, _ = O.rv_nodes(n)
+        V = np.stack([O.koorn_pols(u, v, n) for (u, v) in rv])
+        U = O.rv_umat(n)
+        assert np.abs(U @ V - np.eye(len(V))).max() < 1e-10
+
+
+def test_koornwinder_closed_form_low_degree():
+    # K_{0,0} = sqrt(2); K_{1,0} = P_1^{(0,1)}(1-2v) sqrt(4); K_{1,1} = (2u+v-1) sqrt(12)
+    for (u, v) in [(0.1, 0.2), (0.3, 0.5), (0.0, 1.0)]:
+        P = O.koorn_pols(u, v, 1)
+        x = 1 - 2 * v
+        assert abs(P[0] - math.sqrt(2)) < 1e-15
+        assert abs(P[1] - (-1 + 3 * x) / 2 * 2) < 1e-14
+        assert abs(P[2] - (2 * u + v - 1) * math.sqrt(12)) < 1e-14
+
+
+def _golden_patches(g):
+    n = g["norder"]
+    uvs, _ = O.rv_nodes(n)
+    U = O.rv_umat(n)
+    rr = g["rr"]
+    th = np.pi / 4
+    p = len(uvs)
+    x1 = np.stack([uvs[:, 0] * rr, uvs[:, 1] * rr, 0 * uvs[:, 0]], 1)
+    d1 = np.tile([rr, 0, 0, 0, rr, 0], (p, 1))
+    x2 = np.stack([np.cos(th) * uvs[:, 0] + np.sin(th) * uvs[:, 1] + 2.1,
+                   -np.sin(th) * uvs[:, 0] + np.cos(th) * uvs[:, 1] + 1.1, 0 * uvs[:, 0]], 1)
+    d2 = np.tile([np.cos(th), -np.sin(th), 0, np.sin(th), np.cos(th), 0], (p, 1))
+    return [U @ np.concatenate([x1, d1], 1), U @ np.concatenate([x2, d2], 1)]
+
+
+def test_triaints_golden_adaptive_and_fixed():
+    """test/tria_routs/test_triaintrouts.f:160-170,290,459 -- exact integrals of 1/r."""
+    g = json.loads((GOLD / "triaints_golden.json").read_text())
+    sc = _golden_patches(g)
+    ex = np.array(g["cintex"])
+    got_a, got_f = [], []
+    q, qw = O.xg_unif_nodes(g["nlev_wnodes"], g["nqorder_wnodes"])
+    for t, ip in zip(g["targets"], g["target_patch"]):
+        c, _, _ = O.triaints_adap(g["eps_adaptive"], g["nqorder_adaptive"], g["norder"], sc[ip - 1], np.array(t),
+                                  O.ONE_OVER_R, None)
+        got_a.append(c[:3])
+        got_f.append(O.triaints_wnodes(g["norder"], sc[ip - 1], np.array(t)[None], q, qw, O.ONE_OVER_R, None)[0, :3])
+    err_a = np.sqrt((np.abs(np.array(got_a) - ex) ** 2).sum() / (np.abs(ex) ** 2).sum())
+    err_f = np.sqrt((np.abs(np.array(got_f) - ex) ** 2).sum() / (np.abs(ex) ** 2).sum())
+    assert err_a < g["tol_adaptive"]
+    assert err_f < g["tol_wnodes"]
+    # tighter: this restatement reaches 1e-9 on the adaptive strategy
+    assert err_a < 1e-8
+
+
+def test_kernels_closed_form_and_normal_derivative():
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        src = np.zeros(12)
+        src[:3] = rng.standard_normal(3)
+        n = rng.standard_normal(3)
+        src[9:] = n / np.linalg.norm(n)
+        t = rng.standard_normal(3) * 2
+        zk, a, b = 1.3 + 0.1j, 0.7 - 0.2j, 1.1 + 0.4j
+        r = np.linalg.norm(t - src[:3])
+        d = (t - src[:3]) @ src[9:]
+        want = (b * d * (1 - 1j * zk * r) / r ** 3 + a / r) * np.exp(1j * zk * r) / (4 * np.pi)
+        got = O.kernel(O.HELM_COMB, np.array([zk, a, b]), src, t)
+        assert abs(got - want) < 1e-14 * abs(want)
+        # DLP = d/dn_y of the SLP (central difference along the normal)
+        h = 1e-5
+        sp, sm = src.copy(), src.copy()
+        sp[:3] += h * src[9:]
+        sm[:3] -= h * src[9:]
+        fd = (O.kernel(O.HELM_SLP, np.array([zk, 0, 0]), sp, t) - O.kernel(O.HELM_SLP, np.array([zk, 0, 0]), sm, t)) / (2 * h)
+        dl = O.kernel(O.HELM_DLP, np.array([zk, 0, 0]), src, t)
+        assert abs(fd - dl) < 1e-7 * abs(dl)
+        lap = O.kernel(O.LAP_COMB, np.array([0.5, 2.0]), src, t)
+        assert abs(lap - (2.0 * d / r ** 3 + 0.5 / r) / (4 * np.pi)) < 1e-14 * abs(lap)
+
+
+def _sphere(na, norder):
+    from fmm3dbie_amd import geometry as G
+    return G.get_sphere_npat(1.0, na, (0, 0, 0), norder)
+
+
+def test_self_polar_matches_independent_duffy():
+    """The GPU self rule (oracle restatement) vs a Duffy map with 80x80 Gauss points."""
+    g = _sphere(2, 4)
+    p = 15
+    sc, sv = g["srccoefs"][:p], g["srcvals"][:p]
+    uv, _ = O.rv_nodes(4)
+    i = 1
+    u0 = uv[i]
+    x, w = np.polynomial.legendre.leggauss(60)
+    x = (x + 1) / 2
+    w = w / 2
+    verts = [np.array([0, 0.0]), np.array([1, 0.0]), np.array([0, 1.0])]
+    tot = np.zeros(p, complex)
+    zk = 1.1
+    for e in range(3):
+        A, B = verts[e], verts[(e + 1) % 3]
+        det = abs((A - u0)[0] * (B - A)[1] - (A - u0)[1] * (B - A)[0])
+        for r, wr in zip(x, w):
+            for t, wt in zip(x, w):
+                y = u0 + r * (A - u0 + t * (B - A))
+                P = O.koorn_pols(y[0], y[1], 4)
+                s = P @ sc
+                nrm = np.cross(s[3:6], s[6:9])
+                J = np.linalg.norm(nrm)
+                src = np.concatenate([s, nrm / J])
+                k = O.kernel(O.HELM_COMB, np.array([zk, 1j * zk, 1.0]), src, sv[i, :3])
+                tot += wr * wt * r * det * J * k * P
+    want = O.rv_umat(4).T @ tot
+    got = O.self_polar(4, u0, sc, sv[i], O.HELM_COMB, np.array([zk, 1j * zk, 1.0]), 10, 10)
+    assert np.abs(got - want).max() < 1e-9 * np.abs(want).max()
+
+
+def test_self_polar_vs_reference_adaptive_self_on_consistent_geometry():
+    """zget_adap_self_quad_pt (adap-quads.f:935-1253) and the polar rule agree when the
+    derivative data are the exact derivatives of the xyz expansion."""
+    n = 4
+    uv, _ = O.rv_nodes(n)
+    U = O.rv_umat(n)
+    p = len(uv)
+    sv = np.zeros((p, 12))
+    for l, (u, v) in enumerate(uv):
+        X = np.array([u + 0.2 * v, 0.9 * v, 0.5 * u * u + 0.3 * u * v + 0.4 * v * v])
+        Xu = np.array([1, 0, u + 0.3 * v])
+        Xv = np.array([0.2, 0.9, 0.3 * u + 0.8 * v])
+        nn = np.cross(Xu, Xv)
+        sv[l] = np.concatenate([X, Xu, Xv, nn / np.linalg.norm(nn)])
+    sc = U @ sv[:, :9]
+    for kind, pars in [(O.LAP_SLP, np.array([1.0, 0])), (O.LAP_DLP, np.array([0.0, 1.0])),
+                       (O.HELM_COMB, np.array([2.0, 2j, 1.0]))]:
+        for i in [0, 5, 11]:
+            a = O.self_adap(1e-13, n, uv[i], sc, sv[i], kind, pars, 16)
+            b = O.self_polar(n, uv[i], sc, sv[i], kind, pars, 10, 10)
+            # the adaptive rule stops refining at level 20 (triaintrouts.f:853), which
+            # bounds its own accuracy at the target vertex to ~1e-5 for the DLP
+            assert np.abs(a - b).max() < 5e-5 * np.abs(b).max()
+            assert abs(a.sum() - b.sum()) < 1e-5 * abs(b.sum())
+
+
+def _sph_eig(zk, n):
+    from scipy.special import spherical_jn, spherical_yn
+    j = spherical_jn(n, zk)
+    h = j + 1j * spherical_yn(n, zk)
+    hp = spherical_jn(n, zk, True) + 1j * spherical_yn(n, zk, True)
+    return 1j * zk * j * h, 1j * zk * zk * j * hp + 0.5
+
+
+@pytest.mark.parametrize("helm", [False, True])
+def test_operator_on_sphere_eigenfunction(helm):
+    """alpha S + beta D applied to Y_1 = z on the unit sphere: Laplace S = 1/3, D = -1/6;
+    Helmholtz S_k = i k j_1 h_1, D_k = i k^2 j_1 h_1' + 1/2."""
+    from fmm3dbie_amd import problem
+    # order 6 on a 3x3-per-face cube sphere: the double layer converges at the rate of
+    # the discretized normals (~1e-2 at order 4, ~7e-4 at order 6 on this mesh)
+    g = _sphere(3, 6)
+    zk = 1.1
+    pars = np.array([zk, 1j * zk, 1.0]) if helm else np.array([1.0, 1.0])
+    P = problem.setup(g, 1e-7, helm, pars)
+    w = O.getnearquad(g, g["srcvals"], P["ipatch_id"], P["uvs_targ"], 1e-7, O.HELM_COMB if helm else O.LAP_COMB,
+                      pars, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, 10, 10)
+    sig = g["srcvals"][:, 2].astype(complex if helm else float)
+    pot = O.lpcomp_addsub(int(helm), g, g["srcvals"], pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"],
+                          P["ixyzso"], P["srcover"], P["whtsover"], sig)
+    if helm:
+        s, d = _sph_eig(zk, 1)
+        lam = pars[1] * s + pars[2] * d
+    else:
+        lam = 1.0 / 3 - 1.0 / 6
+    err = np.abs(pot - lam * sig).max() / np.abs(lam * sig).max()
+    assert err < 2e-3
+
+
+def test_dlaran_and_hkrand_exact():
+    """dlaran is the 48-bit LCG x <- a x mod 2^48, a = (494,322,2508,2549) base 4096."""
+    from fmm3dbie_amd import near
+    a = ((494 * 4096 + 322) * 4096 + 2508) * 4096 + 2549
+    for seed in [1, 1002, 3043, 4095]:
+        s = (2 * seed + 1) % 4096
+        x = (a * s) % (1 << 48)
+        want = x / float(1 << 48)
+        assert O.hkrand(seed) == want
+        assert near.hkrand(seed) == want
+
+
+def test_go3_fixture_pins_rv_nodes():
+    """geometries/sphere_192_o03.go3 = refined-cube sphere sampled at order-3 RV nodes."""
+    from fmm3dbie_amd import geometry as G
+    d = np.load(GOLD / "sphere_192_o03.npz")
+    g = G.get_platonic_sphere(2, int(d["norder"]))
+    assert int(d["npatches"]) == len(g["norders"])
+    assert np.abs(d["srcvals"] - g["srcvals"]).max() < 1e-14
+
+
+def test_findnear_matches_bruteforce():
+    from fmm3dbie_amd import near
+    rng = np.random.default_rng(3)
+    cms = rng.standard_normal((101, 3))
+    rads = rng.uniform(0.1, 0.8, 101)
+    targs = rng.standard_normal((1000, 3))
+    r1, c1 = near.findnear(cms, rads, targs)
+    r2, c2 = O.findnear(cms, rads, targs)
+    assert np.array_equal(r1, r2) and np.array_equal(c1, c2)
+
+
+def test_host_setup_matches_oracle():
+    from fmm3dbie_amd import problem
+    g = _sphere(3, 4)
+    zk = 1.1
+    pars = np.array([zk, 1j * zk, 1.0])
+    P = problem.setup(g, 1e-6, True, pars)
+    cms, rads = O.centroid_rads(g["norders"], g["ixyzs"], g["srccoefs"])
+    assert np.abs(cms - P["cms"]).max() < 1e-14 and np.abs(rads - P["rads"]).max() < 1e-14
+    rp, ci = O.findnear(cms, rads * P["rfac"], g["srcvals"])
+    assert np.array_equal(rp, P["row_ptr"]) and np.array_equal(ci, P["col_ind"])
+    nv, _ = O.far_order(1e-6, g, cms, rads, g["srcvals"][:, :3], 0, zk, rp, ci, P["rfac"])
+    assert np.array_equal(nv, P["novers"])
+    so, wo = O.oversample_geom(g, P["novers"], P["ixyzso"])
+    assert np.abs(so - P["srcover"]).max() < 1e-14 and np.abs(wo - P["whtsover"]).max() < 1e-15
+
+
+def test_oracle_gmres_small_dense():
+    """GMRES restatement on a tiny problem vs a dense solve of the same operator."""
+    from fmm3dbie_amd import problem
+    g = _sphere(1, 2)
+    pars = np.array([1.0, 1.0])
+    P = problem.setup(g, 1e-6, False, pars)
+    w = O.getnearquad(g, g["srcvals"], P["ipatch_id"], P["uvs_targ"], 1e-6, O.LAP_COMB, pars, P["row_ptr"],
+                      P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, 10, 10)
+    N = len(g["srcvals"])
+    A = np.zeros((N, N))
+    for j in range(N):
+        e = np.zeros(N)
+        e[j] = 1
+        A[:, j] = O.lpcomp_addsub(0, g, g["srcvals"], pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"],
+                                  P["ixyzso"], P["srcover"], P["whtsover"], e)
+    A += 0.5 * np.eye(N)
+    rhs = problem.point_source(g["srcvals"][:, :3], [0.1, 0.2, -0.3])
+    x, nit, errs, rres = O.gmres(0, g, P["wts"], pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"],
+                                 P["ixyzso"], P["srcover"], P["whtsover"], 0.5, rhs, 60, 1e-12)
+    xd = np.linalg.solve(A, rhs)
+    assert np.abs(x - xd).max() < 1e-9 * np.abs(xd).max()
+    assert rres < 1e-10
