@@ -76,15 +76,15 @@ __device__ __forceinline__ void pair_kernel(T dx, T dy, T dz, T nx, T ny, T nz, 
     rinv = __frsqrt_rn(r2);
     r = r2 * rinv;
   } else {
-    r = sqrt(r2);
-    rinv = 1.0 / r;
+    rinv = rsqrt_nr(r2);
+    r = r2 * rinv;
   }
   const T d = dx * nx + dy * ny + dz * nz;
   T s, c;
   if constexpr (sizeof(T) == 4) {
     __sincosf(kr * r, &s, &c);
   } else {
-    sincos(kr * r, &s, &c);
+    sincos_cw(kr * r, &s, &c);
   }
   T mag = rinv;
   if (ki != (T)0) {
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
                                                 const V2<T>* __restrict__ mstr, int64_t ldstr,
                                                 const int64_t* __restrict__ row_ptr,
                                                 const int* __restrict__ near_sorted, T kr, T ki, T thresh2,
-                                                int accumulate, V2<T>* __restrict__ pot, int64_t ldpot) {
+                                                int64_t chunk, V2<T>* __restrict__ part, int64_t ldpart) {
   __shared__ T lx[TPB], ly[TPB], lz[TPB], lnx[TPB], lny[TPB], lnz[TPB];
   __shared__ V2<T> lq[NRHS][TPB], lm[NRHS][TPB];
   __shared__ int lp[TPB];
@@ -121,16 +121,20 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
   T accr[NRHS], acci[NRHS];
 #pragma unroll
   for (int r = 0; r < NRHS; r++) { accr[r] = 0; acci[r] = 0; }
+  const int64_t jbeg = blockIdx.y * chunk;
+  const int64_t jend = jbeg + chunk < nsrc ? jbeg + chunk : nsrc;
   int64_t nptr = 0, nend = 0;
   int nextnear = 0x7fffffff;
-  if (MASK && act) {
+  if (MASK && act && jbeg < jend) {
     nptr = row_ptr[i] - 1;
     nend = row_ptr[i + 1] - 1;
+    const int p0 = spatch[jbeg];
+    while (nptr < nend && near_sorted[nptr] < p0) nptr++;
     nextnear = nptr < nend ? near_sorted[nptr] : 0x7fffffff;
   }
-  for (int64_t base = 0; base < nsrc; base += TPB) {
+  for (int64_t base = jbeg; base < jend; base += TPB) {
     const int64_t j = base + threadIdx.x;
-    if (j < nsrc) {
+    if (j < jend) {
       lx[threadIdx.x] = sx[j]; ly[threadIdx.x] = sy[j]; lz[threadIdx.x] = sz[j];
       lnx[threadIdx.x] = snx[j]; lny[threadIdx.x] = sny[j]; lnz[threadIdx.x] = snz[j];
 #pragma unroll
@@ -141,7 +145,7 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
       if (MASK) lp[threadIdx.x] = spatch[j];
     }
     __syncthreads();
-    const int cnt = nsrc - base < TPB ? (int)(nsrc - base) : TPB;
+    const int cnt = jend - base < TPB ? (int)(jend - base) : TPB;
     for (int k = 0; k < cnt; k++) {
       if (MASK) {
         const int pk = lp[k];
@@ -168,15 +172,24 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
   }
   if (act) {
 #pragma unroll
-    for (int r = 0; r < NRHS; r++) {
-      V2<T> o = {(T)(accr[r] * (T)OVER4PI), (T)(acci[r] * (T)OVER4PI)};
-      if (accumulate) {
-        o.x += pot[r * ldpot + i].x;
-        o.y += pot[r * ldpot + i].y;
-      }
-      pot[r * ldpot + i] = o;
-    }
+    for (int r = 0; r < NRHS; r++) part[(int64_t)blockIdx.y * ldpart + r * ntarg + i] = {accr[r], acci[r]};
   }
+}
+
+// pot[r][i] = (sum over source splits of part) / (4 pi)
+template <typename T>
+__global__ void k_farsum_reduce(int64_t ntarg, int nrhs, int nsplit, const V2<T>* __restrict__ part,
+                                int64_t ldpart, V2<T>* __restrict__ pot, int64_t ldpot) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (i >= ntarg || r >= nrhs) return;
+  T sr = 0, si = 0;
+  for (int sp = 0; sp < nsplit; sp++) {
+    const V2<T> v = part[(int64_t)sp * ldpart + r * ntarg + i];
+    sr += v.x;
+    si += v.y;
+  }
+  pot[r * ldpot + i] = {(T)(sr * (T)OVER4PI), (T)(si * (T)OVER4PI)};
 }
 
 // Near subtraction (ADDSUB mode): one wavefront per target, lanes over the oversampled
@@ -480,6 +493,7 @@ int apply_plan_build(ApplyPlan* P, const ApplyPlanInput& in, hipStream_t s) {
 static int ensure_ws(ApplyPlan* P, int nrhs) {
   if (P->ws_cap >= nrhs) return 0;
   if (P->q64) { (void)hipFree(P->q64); (void)hipFree(P->m64); (void)hipFree(P->q32); (void)hipFree(P->m32); }
+  if (P->part64) { (void)hipFree(P->part64); (void)hipFree(P->part32); }
   if (P->sig64c) (void)hipFree(P->sig64c);
   if (P->pot64c) (void)hipFree(P->pot64c);
   const size_t NO = P->nptso;
@@ -489,17 +503,21 @@ static int ensure_ws(ApplyPlan* P, int nrhs) {
   P->m32 = dal<float>(2 * NO * nrhs);
   P->sig64c = dal<double>(2 * (size_t)P->npts);
   P->pot64c = dal<double>(2 * (size_t)P->npts);
-  if (!P->q64 || !P->m64 || !P->q32 || !P->m32 || !P->sig64c || !P->pot64c) return -2;
+  // partial far sums: far_split_cap splits x up to 8 batched columns x npts
+  const size_t npart = (size_t)P->far_split_cap * 8 * P->npts;
+  P->part64 = dal<double>(2 * npart);
+  P->part32 = dal<float>(2 * npart);
+  if (!P->q64 || !P->m64 || !P->q32 || !P->m32 || !P->sig64c || !P->pot64c || !P->part64 || !P->part32) return -2;
   P->ws_cap = nrhs;
   return 0;
 }
 
 void apply_plan_free_ws(ApplyPlan* P) {
-  void* ps[] = {P->q64, P->m64, P->q32, P->m32, P->sig64c, P->pot64c};
+  void* ps[] = {P->q64, P->m64, P->q32, P->m32, P->sig64c, P->pot64c, P->part64, P->part32};
   for (void* p : ps)
     if (p) (void)hipFree(p);
-  P->q64 = P->m64 = P->sig64c = P->pot64c = nullptr;
-  P->q32 = P->m32 = nullptr;
+  P->q64 = P->m64 = P->sig64c = P->pot64c = P->part64 = nullptr;
+  P->q32 = P->m32 = P->part32 = nullptr;
   P->ws_cap = 0;
 }
 
@@ -515,17 +533,35 @@ static void launch_far(ApplyPlan* P, bool mask, const V2<T>* q, const V2<T>* m, 
     src = (const T* const*)P->src32;
     trg = (const T* const*)P->trg32;
   }
+  if (P->npts == 0) return;
+  if (P->nptso == 0) {
+    for (int r = 0; r < NRHS; r++) (void)hipMemsetAsync(pot + r * ldpot, 0, sizeof(V2<T>) * P->npts, s);
+    return;
+  }
   const T kr = (T)(P->helm ? P->zk[0] : 0.0), ki = (T)(P->helm ? P->zk[1] : 0.0);
   const T th2 = (T)(P->thresh * P->thresh);
   const unsigned g = nblk(P->npts, TPB);
+  // split the sources so that the grid holds >= ~8 workgroups per CU
+  const int64_t ntiles = (P->nptso + TPB - 1) / TPB;
+  int64_t nsplit = (P->far_wg_target + g - 1) / g;
+  if (nsplit < 1) nsplit = 1;
+  if (nsplit > ntiles) nsplit = ntiles;
+  if (nsplit > P->far_split_cap) nsplit = P->far_split_cap;
+  const int64_t chunk = ((ntiles + nsplit - 1) / nsplit) * TPB;
+  nsplit = (P->nptso + chunk - 1) / chunk;
+  V2<T>* part = (V2<T>*)(sizeof(T) == 8 ? (void*)P->part64 : (void*)P->part32);
+  const int64_t ldpart = (int64_t)NRHS * P->npts;
+  dim3 grid(g, (unsigned)nsplit);
   if (mask)
-    k_farsum<T, true, NRHS><<<g, TPB, 0, s>>>(P->npts, trg[0], trg[1], trg[2], P->nptso, src[0], src[1], src[2],
-                                              src[3], src[4], src[5], P->opatch, q, m, ldstr, P->row_ptr,
-                                              P->near_sorted, kr, ki, th2, 0, pot, ldpot);
+    k_farsum<T, true, NRHS><<<grid, TPB, 0, s>>>(P->npts, trg[0], trg[1], trg[2], P->nptso, src[0], src[1], src[2],
+                                                 src[3], src[4], src[5], P->opatch, q, m, ldstr, P->row_ptr,
+                                                 P->near_sorted, kr, ki, th2, chunk, part, ldpart);
   else
-    k_farsum<T, false, NRHS><<<g, TPB, 0, s>>>(P->npts, trg[0], trg[1], trg[2], P->nptso, src[0], src[1], src[2],
-                                               src[3], src[4], src[5], P->opatch, q, m, ldstr, P->row_ptr,
-                                               P->near_sorted, kr, ki, th2, 0, pot, ldpot);
+    k_farsum<T, false, NRHS><<<grid, TPB, 0, s>>>(P->npts, trg[0], trg[1], trg[2], P->nptso, src[0], src[1],
+                                                  src[2], src[3], src[4], src[5], P->opatch, q, m, ldstr,
+                                                  P->row_ptr, P->near_sorted, kr, ki, th2, chunk, part, ldpart);
+  k_farsum_reduce<T><<<dim3(nblk(P->npts, 256), NRHS), 256, 0, s>>>(P->npts, NRHS, (int)nsplit, part, ldpart, pot,
+                                                                     ldpot);
 }
 
 // sigma/pot: complex (interleaved) device arrays of length npts * nrhs, leading dim npts

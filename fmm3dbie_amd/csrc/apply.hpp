@@ -44,8 +44,11 @@ struct ApplyPlan {
   float* wnear32 = nullptr;   // complex64 copy for the mixed-precision path
   // workspace
   int ws_cap = 0;
-  double *q64 = nullptr, *m64 = nullptr, *sig64c = nullptr, *pot64c = nullptr;
-  float *q32 = nullptr, *m32 = nullptr;
+  double *q64 = nullptr, *m64 = nullptr, *sig64c = nullptr, *pot64c = nullptr, *part64 = nullptr;
+  float *q32 = nullptr, *m32 = nullptr, *part32 = nullptr;
+  // far-sum launch shape: target workgroups x source splits >= far_wg_target
+  int64_t far_wg_target = 2048;
+  int far_split_cap = 32;
   std::vector<void*> owned;
   ~ApplyPlan();
 };

@@ -30,6 +30,10 @@
 #include "quad.hpp"
 
 __constant__ KoornConst g_koorn;
+extern __shared__ double fmm_dyn_lds[];
+
+// library sin/cos for the arguments outside the reduced-range fast path
+__device__ __attribute__((noinline)) void sincos_large(double x, double* sn, double* cs) { sincos(x, sn, cs); }
 
 namespace {
 
@@ -186,7 +190,7 @@ __global__ void __launch_bounds__(128) k_fixed(int nf, const double* __restrict_
                                                const int2* __restrict__ fixed_list, KPar kp,
                                                double* __restrict__ wnear) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
-  extern __shared__ double lds[];
+  double* lds = fmm_dyn_lds;
   const int ip = blockIdx.x;
   const int a = off[ip], b = off[ip + 1];
   if (a == b) return;
@@ -327,7 +331,6 @@ __global__ void __launch_bounds__(64) k_adaptive(int npairs, const int2* __restr
                                                  int* __restrict__ overflow, double* __restrict__ wnear) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
   constexpr int ES = 6 + 2 * NP;  // stack entry: x0,y0,dx,dy,level,pad, moments
-  __shared__ double sc[9 * NP];
   __shared__ double buf[WAVE * (NP + 2)];
   __shared__ int sh_idx;
   const int lane = threadIdx.x;
@@ -336,14 +339,14 @@ __global__ void __launch_bounds__(64) k_adaptive(int npairs, const int2* __restr
   for (;;) {
     if (lane == 0) sh_idx = atomicAdd(counter, 1);
     __syncthreads();
-    const int idx = sh_idx;
+    // wave-uniform from here: the pair, its patch and the patch coefficients are read
+    // through scalar loads into SGPRs
+    const int idx = __builtin_amdgcn_readfirstlane(sh_idx);
     __syncthreads();
     if (idx >= npairs) break;
     const int2 jt = list[idx];
     const int64_t ip = col_ind[jt.x] - 1;
-    const double* gsc = srccoefs + 9 * (ixyzs[ip] - 1);
-    for (int i = lane; i < 9 * NP; i += WAVE) sc[i] = gsc[i];
-    __syncthreads();
+    const double* sc = srccoefs + 9 * (ixyzs[ip] - 1);
     const double* tp = targs + ndtarg * (int64_t)jt.y;
     const double tg[3] = {tp[0], tp[1], tp[2]};
     const double eps = epsp_patch[ip];
@@ -366,12 +369,20 @@ __global__ void __launch_bounds__(64) k_adaptive(int npairs, const int2* __restr
       const double cy[4] = {y0, y0, y0 + hy, y0 + hy};
       const double cdx[4] = {hx, hx, hx, -hx};
       const double cdy[4] = {hy, hy, hy, -hy};
-      double mc[4];
       double ssum = 0.0;
-#pragma unroll
+      // the four children go to stack slots sp..sp+3 (committed below only if refined)
+      const bool room = sp + 4 <= STACK_CAP;
+#pragma unroll 1
       for (int c = 0; c < 4; c++) {
-        mc[c] = tri_moments<N, HELM>(buf, sc, cx[c], cy[c], cdx[c], cdy[c], da, kq, xg, tg, kp);
-        ssum += mc[c];
+        const double mcc = tri_moments<N, HELM>(buf, sc, cx[c], cy[c], cdx[c], cdy[c], da, kq, xg, tg, kp);
+        ssum += mcc;
+        if (room) {
+          double* ne = stack + (size_t)(sp + c) * ES;
+          if (lane == 0) {
+            ne[0] = cx[c]; ne[1] = cy[c]; ne[2] = cdx[c]; ne[3] = cdy[c]; ne[4] = (double)(level + 1);
+          }
+          if (lane < NC) ne[6 + lane] = mcc;
+        }
       }
       nref++;
       // error = max_j |sum children - parent| (complex modulus)
@@ -384,15 +395,7 @@ __global__ void __launch_bounds__(64) k_adaptive(int npairs, const int2* __restr
         err = lane < NP ? fabs(dlt) : 0.0;
       }
       err = wave_max(err);
-      if (err > eps && level + 1 <= NLMAX && sp + 4 <= STACK_CAP) {
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-          double* ne = stack + (size_t)(sp + c) * ES;
-          if (lane == 0) {
-            ne[0] = cx[c]; ne[1] = cy[c]; ne[2] = cdx[c]; ne[3] = cdy[c]; ne[4] = (double)(level + 1);
-          }
-          if (lane < NC) ne[6 + lane] = mc[c];
-        }
+      if (err > eps && level + 1 <= NLMAX && room) {
         sp += 4;
       } else {
         if (err > eps && level + 1 <= NLMAX && lane == 0) atomicAdd(overflow, 1);
@@ -417,16 +420,13 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
                                              const double* __restrict__ umat, KPar kp, int nr, int nt,
                                              double tau_panel, double* __restrict__ wnear) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
-  __shared__ double sc[9 * NP];
   __shared__ double buf[WAVE * (NP + 2)];
   const int lane = threadIdx.x;
   const int idx = blockIdx.x;
   if (idx >= nself) return;
   const int2 jt = list[idx];
   const int64_t ip = col_ind[jt.x] - 1;
-  const double* gsc = srccoefs + 9 * (ixyzs[ip] - 1);
-  for (int i = lane; i < 9 * NP; i += WAVE) sc[i] = gsc[i];
-  __syncthreads();
+  const double* sc = srccoefs + 9 * (ixyzs[ip] - 1);
   const double* tp = targs + ndtarg * (int64_t)jt.y;
   const double tg[3] = {tp[0], tp[1], tp[2]};
   const double u0 = uvs_targ[2 * (int64_t)jt.y], v0 = uvs_targ[2 * (int64_t)jt.y + 1];
