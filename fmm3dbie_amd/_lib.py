@@ -12,6 +12,8 @@ vp = C.c_void_p
 i32 = C.c_int
 i64 = C.c_int64
 f64 = C.c_double
+f32 = C.c_float
+u32 = C.c_uint
 
 # argument signatures of the exported symbols (include/fmm3dbie_amd.h)
 SIGS = {
@@ -22,7 +24,11 @@ SIGS = {
     "get_rfacs_": ([vp, vp, vp, vp], None),
     "fmm3dbie_last_error": ([], i32),
     "fmm3dbie_nearquad_dev": ([i32, i32, i32, i64, vp, vp, vp, i64, i64, vp, vp, vp, f64, vp, i64, vp, vp, vp,
-                               f64, vp, i32, f64, vp, vp, vp, vp], i32),
+                               f64, vp, i32, f64, vp, vp, vp, vp, vp, vp, vp], i32),
+    "fmm3dbie_mlp_nparam": ([], i32),
+    "fmm3dbie_mlp_grad": ([vp, vp, vp, i64, vp, vp, vp], i32),
+    "fmm3dbie_mlp_train": ([vp, vp, vp, i64, i32, i32, f32, u32, vp, vp], i32),
+    "fmm3dbie_mlp_predict": ([vp, vp, i64, vp, vp, vp], i32),
     "fmm3dbie_apply_plan_create": ([i32, i64, vp, vp, i64, vp, vp, i64, vp, vp, vp, i64, vp, i32, vp, i64, vp, vp,
                                     vp, vp], vp),
     "fmm3dbie_apply_plan_apply": ([vp, i32, i32, vp, vp, i32, vp, vp], i32),

@@ -16,6 +16,7 @@
 #include "apply.hpp"
 #include "gmres.hpp"
 #include "host_basis.hpp"
+#include "learned.hpp"
 #include "quad.hpp"
 
 namespace {
@@ -358,7 +359,8 @@ int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatc
                           const double* targs, const int64_t* ipatch_id, const double* uvs_targ, double eps,
                           const double* kpar6, int64_t nnz, const int64_t* row_ptr, const int64_t* col_ind,
                           const int64_t* iquad, double rfac0, double* wnear, int self_n, double self_tau_panel,
-                          void* stream, double* stats, int* nref_host, int* class_host) {
+                          void* stream, double* stats, int* nref_host, int* class_host, const float* mlp_params,
+                          float* feat_out, int* pred_host) {
   NearQuadArgs A = {};
   A.helm = helm != 0;
   A.norder = norder;
@@ -377,11 +379,18 @@ int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatc
   A.want_nref = nref_host != nullptr;
   A.nref_host = nref_host;
   A.class_host = class_host;
+  A.mlp_params = mlp_params;
+  A.feat_out = feat_out;
+  A.pred_host = pred_host;
   NearQuadStats st = {};
   const int rc = fmm_nearquad_device(A, &st, (hipStream_t)stream);
   if (stats) {
-    const double v[12] = {(double)st.nself, (double)st.nadap, (double)st.nfixed, (double)st.overflow,
-                          (double)st.nf, (double)st.kq, st.ms_prep, st.ms_fixed, st.ms_adap, st.ms_self, 0, 0};
+    const double v[20] = {(double)st.nself,     (double)st.nadap,     (double)st.nfixed, (double)st.overflow,
+                          (double)st.nf,        (double)st.kq,        st.ms_prep,        st.ms_fixed,
+                          st.ms_adap,           st.ms_self,           st.ms_tables,      st.ms_select,
+                          st.ms_tree,           st.ms_shallow,        (double)st.nshallow0,
+                          (double)st.nshallow1, (double)st.nfallback, 0,                 0,
+                          0};
     std::memcpy(stats, v, sizeof(v));
   }
   g_last_error = rc;
@@ -569,6 +578,33 @@ void lap_comb_dir_solver_guru_(const int64_t* npatches, const int64_t* norders, 
   solver_guru_host(false, *npatches, norders, ixyzs, iptype, *npts, srcvals, kp, *numit, *ifinout, rhs, *nnz,
                    row_ptr, col_ind, iquad, *nquad, wnear, novers, *nptso, ixyzso, srcover, whtsover, *eps_gmres,
                    niter, errs, rres, soln);
+}
+
+// ---------------------------------------------------------------- learned selector
+int fmm3dbie_mlp_nparam() { return MLP_NPARAM; }
+
+int fmm3dbie_mlp_grad(const float* params, const float* X, const int* y, int64_t n, float* grad, float* loss,
+                      void* stream) {
+  const int rc = n > 0 ? mlp_grad_device(params, X, y, n, grad, loss, (hipStream_t)stream) : -6;
+  g_last_error = rc;
+  return rc;
+}
+
+int fmm3dbie_mlp_train(float* params, const float* X, const int* y, int64_t n, int epochs, int batch, float lr,
+                       unsigned seed, float* loss_hist, void* stream) {
+  int rc = -6;
+  if (n >= batch && epochs > 0)
+    rc = mlp_train_device(params, X, y, n, epochs, batch, lr, seed, loss_hist, (hipStream_t)stream);
+  else
+    fprintf(stderr, "fmm3dbie_amd: mlp_train needs n >= batch and epochs > 0\n");
+  g_last_error = rc;
+  return rc;
+}
+
+int fmm3dbie_mlp_predict(const float* params, const float* X, int64_t n, int* cls, float* logits, void* stream) {
+  const int rc = mlp_predict_device(params, X, n, cls, logits, (hipStream_t)stream);
+  g_last_error = rc;
+  return rc;
 }
 
 }  // extern "C"

@@ -27,6 +27,7 @@
 
 #include "common.hpp"
 #include "host_basis.hpp"
+#include "learned.hpp"
 #include "quad.hpp"
 
 __constant__ KoornConst g_koorn;
@@ -63,7 +64,7 @@ template <int N>
 __global__ void k_patch_prep(int64_t npatches, const int64_t* __restrict__ ixyzs,
                              const double* __restrict__ srccoefs, const double* __restrict__ srcvals,
                              const double* __restrict__ rvw, double eps_adap, double* __restrict__ cms,
-                             double* __restrict__ rads, double* __restrict__ epsp) {
+                             double* __restrict__ rads, double* __restrict__ epsp, double* __restrict__ area) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
   const int64_t ip = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (ip >= npatches) return;
@@ -91,15 +92,17 @@ __global__ void k_patch_prep(int64_t npatches, const int64_t* __restrict__ ixyzs
   cms[3 * ip + 1] = c[1];
   cms[3 * ip + 2] = c[2];
   rads[ip] = sqrt(r);
-  double rsc = 0.0;
+  double rsc = 0.0, ar = 0.0;
   for (int i = 0; i < NP; i++) {
     const double* s = srcvals + 12 * (i0 + i);
     const double nx = s[4] * s[8] - s[5] * s[7], ny = s[5] * s[6] - s[3] * s[8],
                  nz = s[3] * s[7] - s[4] * s[6];
     const double rr = (nx * nx + ny * ny + nz * nz) * rvw[i] * rvw[i];
     if (i == 0 || rr < rsc) rsc = rr;
+    ar += sqrt(nx * nx + ny * ny + nz * nz) * rvw[i];
   }
   epsp[ip] = eps_adap * pow(rsc, 0.25);
+  area[ip] = ar;
 }
 
 // ------------------------------------------------------------------ 2. classify
@@ -317,8 +320,85 @@ __device__ __forceinline__ double tri_moments(double* buf, const double* sc, dou
   return total;
 }
 
+// Depth 0..2 of the refinement tree (1 + 4 + 16 triangles, numbered root 0, children
+// of the root 1..4, children of t in 1..4 at 5 + 4(t-1) + c) are the same for every
+// target of a patch, as in the reference's per-patch tree (triaintrouts.f:939-1033).
+// Their node geometry is computed once per patch into HBM (structure of arrays:
+// x, y, z, nx, ny, nz, |J|*da*w) and the basis values at those nodes once per order.
+constexpr int NCACHE_TRI = 21;
+
+template <int N>
+__global__ void k_tree_cache(int64_t npatches, int kq, const double* __restrict__ tuv, const double* __restrict__ twq,
+                             const double* __restrict__ ptab, const int64_t* __restrict__ ixyzs,
+                             const double* __restrict__ srccoefs, double* __restrict__ cache) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  const int64_t ip = blockIdx.x;
+  const int nn = NCACHE_TRI * kq;
+  const double* sc = srccoefs + 9 * (ixyzs[ip] - 1);
+  double* out = cache + (size_t)ip * 7 * nn;
+  for (int q = threadIdx.x; q < nn; q += blockDim.x) {
+    double g[12];
+    for (int m = 0; m < 9; m++) g[m] = 0.0;
+    for (int l = 0; l < NP; l++) {
+      const double pl = ptab[(size_t)q * NP + l];
+      for (int m = 0; m < 9; m++) g[m] = fma(sc[9 * l + m], pl, g[m]);
+    }
+    const double jac = geom_finish(g);
+    out[0 * nn + q] = g[0];
+    out[1 * nn + q] = g[1];
+    out[2 * nn + q] = g[2];
+    out[3 * nn + q] = g[9];
+    out[4 * nn + q] = g[10];
+    out[5 * nn + q] = g[11];
+    out[6 * nn + q] = jac * twq[q];
+  }
+}
+
+// moments over cached tree triangle `tri`: lane q evaluates the kernel at node q, lane c
+// sums column c of (f_q * P_j(q)) with the basis values read from the shared table
 template <int N, bool HELM>
-__global__ void __launch_bounds__(64) k_adaptive(int npairs, const int2* __restrict__ list,
+__device__ __forceinline__ double tri_moments_cached(double* buf, const double* __restrict__ cpatch, int nn,
+                                                     const double* __restrict__ ptab, int tri, int kq,
+                                                     const double* tg, const KPar& kp) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  constexpr int NC = HELM ? 2 * NP : NP;
+  const int lane = threadIdx.x;
+  double total = 0.0;
+  for (int base = 0; base < kq; base += WAVE) {
+    const int q = base + lane;
+    if (q < kq) {
+      const int node = tri * kq + q;
+      double g[12];
+      g[0] = cpatch[node];
+      g[1] = cpatch[nn + node];
+      g[2] = cpatch[2 * nn + node];
+      g[9] = cpatch[3 * nn + node];
+      g[10] = cpatch[4 * nn + node];
+      g[11] = cpatch[5 * nn + node];
+      const double w = cpatch[6 * nn + node];
+      double vr, vi;
+      kern_eval<HELM>(tg, g, kp, vr, vi);
+      buf[2 * lane] = vr * w;
+      buf[2 * lane + 1] = vi * w;
+    }
+    const int nrows = kq - base < WAVE ? kq - base : WAVE;
+    __syncthreads();
+    if (lane < NC) {
+      const int j = lane % NP, part = lane / NP;
+      const double* pcol = ptab + (size_t)(tri * kq + base) * NP + j;
+      double sacc = 0.0;
+#pragma unroll 4
+      for (int r = 0; r < nrows; r++) sacc = fma(buf[2 * r + part], pcol[(size_t)r * NP], sacc);
+      total += sacc;
+    }
+    __syncthreads();
+  }
+  return total;
+}
+
+template <int N, bool HELM>
+__global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restrict__ list, int lpt,
+                                                 const int2* __restrict__ fb_list, const int* __restrict__ fb_n,
                                                  int* __restrict__ counter, int kq, const double* __restrict__ xg,
                                                  const int64_t* __restrict__ col_ind,
                                                  const int64_t* __restrict__ ixyzs,
@@ -328,7 +408,8 @@ __global__ void __launch_bounds__(64) k_adaptive(int npairs, const int2* __restr
                                                  const double* __restrict__ epsp_patch,
                                                  const double* __restrict__ umat, KPar kp,
                                                  double* __restrict__ stack_all, int* __restrict__ nref_out,
-                                                 int* __restrict__ overflow, double* __restrict__ wnear) {
+                                                 int* __restrict__ overflow, double* __restrict__ wnear,
+                                                 const double* __restrict__ cache, const double* __restrict__ ptab) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
   constexpr int ES = 6 + 2 * NP;  // stack entry: x0,y0,dx,dy,level,pad, moments
   __shared__ double buf[WAVE * (NP + 2)];
@@ -336,6 +417,9 @@ __global__ void __launch_bounds__(64) k_adaptive(int npairs, const int2* __restr
   const int lane = threadIdx.x;
   double* stack = stack_all + (size_t)blockIdx.x * STACK_CAP * ES;
   constexpr int NC = HELM ? 2 * NP : NP;
+  // work = the list (taken from its end when lpt: longest predicted first) followed by
+  // the pairs the cached-tree kernels sent back
+  const int npairs = nlist + (fb_n ? *fb_n : 0);
   for (;;) {
     if (lane == 0) sh_idx = atomicAdd(counter, 1);
     __syncthreads();
@@ -344,16 +428,18 @@ __global__ void __launch_bounds__(64) k_adaptive(int npairs, const int2* __restr
     const int idx = __builtin_amdgcn_readfirstlane(sh_idx);
     __syncthreads();
     if (idx >= npairs) break;
-    const int2 jt = list[idx];
+    const int2 jt = idx < nlist ? list[lpt ? nlist - 1 - idx : idx] : fb_list[idx - nlist];
     const int64_t ip = col_ind[jt.x] - 1;
     const double* sc = srccoefs + 9 * (ixyzs[ip] - 1);
     const double* tp = targs + ndtarg * (int64_t)jt.y;
     const double tg[3] = {tp[0], tp[1], tp[2]};
     const double eps = epsp_patch[ip];
-    // root triangle
-    double mroot = tri_moments<N, HELM>(buf, sc, 0.0, 0.0, 1.0, 1.0, 1.0, kq, xg, tg, kp);
+    const int nn = NCACHE_TRI * kq;
+    const double* cpatch = cache + (size_t)ip * 7 * nn;
+    // root triangle (cached tree id 0)
+    double mroot = tri_moments_cached<N, HELM>(buf, cpatch, nn, ptab, 0, kq, tg, kp);
     int sp = 0;
-    if (lane < 5) stack[lane] = lane < 2 ? 0.0 : (lane < 4 ? 1.0 : 0.0);
+    if (lane < 6) stack[lane] = lane < 2 ? 0.0 : (lane < 4 ? 1.0 : 0.0);  // level 0, tree id 0
     if (lane < NC) stack[6 + lane] = mroot;
     sp = 1;
     double result = 0.0;
@@ -363,6 +449,7 @@ __global__ void __launch_bounds__(64) k_adaptive(int npairs, const int2* __restr
       const double* e = stack + (size_t)sp * ES;
       const double x0 = e[0], y0 = e[1], dx = e[2], dy = e[3];
       const int level = (int)e[4];
+      const int tid = (int)e[5];  // cached tree id, -1 below depth 2
       const double mpar = lane < NC ? e[6 + lane] : 0.0;
       const double hx = 0.5 * dx, hy = 0.5 * dy, da = hx * hx;  // child area ratio 4^-(level+1)
       const double cx[4] = {x0, x0 + hx, x0, x0 + hx};
@@ -374,12 +461,16 @@ __global__ void __launch_bounds__(64) k_adaptive(int npairs, const int2* __restr
       const bool room = sp + 4 <= STACK_CAP;
 #pragma unroll 1
       for (int c = 0; c < 4; c++) {
-        const double mcc = tri_moments<N, HELM>(buf, sc, cx[c], cy[c], cdx[c], cdy[c], da, kq, xg, tg, kp);
+        const int ctid = tid < 0 ? -1 : (tid == 0 ? 1 + c : (tid <= 4 ? 5 + 4 * (tid - 1) + c : -1));
+        const double mcc = ctid >= 0
+                               ? tri_moments_cached<N, HELM>(buf, cpatch, nn, ptab, ctid, kq, tg, kp)
+                               : tri_moments<N, HELM>(buf, sc, cx[c], cy[c], cdx[c], cdy[c], da, kq, xg, tg, kp);
         ssum += mcc;
         if (room) {
           double* ne = stack + (size_t)(sp + c) * ES;
           if (lane == 0) {
             ne[0] = cx[c]; ne[1] = cy[c]; ne[2] = cdx[c]; ne[3] = cdy[c]; ne[4] = (double)(level + 1);
+            ne[5] = (double)(ctid >= 0 && ctid <= 4 ? ctid : -1);
           }
           if (lane < NC) ne[6 + lane] = mcc;
         }
@@ -405,6 +496,277 @@ __global__ void __launch_bounds__(64) k_adaptive(int npairs, const int2* __restr
     }
     if (nref_out && lane == 0) nref_out[jt.x] = nref;
     store_weights<NP, HELM>(buf, result, umat, wnear, iquad[jt.x] - 1);
+  }
+}
+
+// ------------------------------------------------------------------ 4b. learned path
+// Selector features of an adaptive pair (target x, source patch with centroid c, radius
+// rad, area A, nodes y_i with unit normals n_i at RV (u_i, v_i)):
+//   |x-c|/rad, |(x-c).n_mean|/rad, min_i |x-y_i|/rad, |(x-y_*).n_*|/rad,
+//   min(u_*, v_*, 1-u_*-v_*), log10(epsp), |k| rad, rad^2/A        (* = nearest node)
+template <int N>
+__global__ void k_features(int nadap, const int2* __restrict__ list, const int64_t* __restrict__ col_ind,
+                           const int64_t* __restrict__ ixyzs, const double* __restrict__ srcvals, int64_t ndtarg,
+                           const double* __restrict__ targs, const double* __restrict__ cms,
+                           const double* __restrict__ rads, const double* __restrict__ area,
+                           const double* __restrict__ epsp, const double* __restrict__ rvuv, double kabs,
+                           float* __restrict__ X, float* __restrict__ feat_full) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nadap) return;
+  const int2 jt = list[idx];
+  const int64_t ip = col_ind[jt.x] - 1, i0 = ixyzs[ip] - 1;
+  const double* tp = targs + ndtarg * (int64_t)jt.y;
+  const double x = tp[0], y = tp[1], z = tp[2];
+  const double rad = rads[ip];
+  const double cx = x - cms[3 * ip], cy = y - cms[3 * ip + 1], cz = z - cms[3 * ip + 2];
+  double mx = 0.0, my = 0.0, mz = 0.0, dmin = 0.0, hn = 0.0;
+  int imin = 0;
+  for (int i = 0; i < NP; i++) {
+    const double* sv = srcvals + 12 * (i0 + i);
+    mx += sv[9];
+    my += sv[10];
+    mz += sv[11];
+    const double dx = x - sv[0], dy = y - sv[1], dz = z - sv[2];
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (i == 0 || d2 < dmin) {
+      dmin = d2;
+      imin = i;
+      hn = fabs(dx * sv[9] + dy * sv[10] + dz * sv[11]);
+    }
+  }
+  const double mn = sqrt(mx * mx + my * my + mz * mz);
+  const double h = mn > 0.0 ? fabs(cx * mx + cy * my + cz * mz) / mn : 0.0;
+  const double u = rvuv[2 * imin], v = rvuv[2 * imin + 1];
+  const double edge = fmin(fmin(u, v), 1.0 - u - v);
+  float f[8];
+  f[0] = (float)(sqrt(cx * cx + cy * cy + cz * cz) / rad);
+  f[1] = (float)(h / rad);
+  f[2] = (float)(sqrt(dmin) / rad);
+  f[3] = (float)(hn / rad);
+  f[4] = (float)edge;
+  f[5] = (float)log10(epsp[ip]);
+  f[6] = (float)(kabs * rad);
+  f[7] = (float)(rad * rad / area[ip]);
+  for (int k = 0; k < 8; k++) X[8 * (int64_t)idx + k] = f[k];
+  if (feat_full)
+    for (int k = 0; k < 8; k++) feat_full[8 * (int64_t)jt.x + k] = f[k];
+}
+
+// bucket key = predicted class * npatches + patch: classes ascending, patches grouped
+__global__ void k_bucket_count(int n, const int2* __restrict__ list, const int* __restrict__ pred,
+                               const int64_t* __restrict__ col_ind, int64_t npat, int* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicAdd(cnt + pred[i] * npat + (col_ind[list[i].x] - 1), 1);
+}
+
+__global__ void k_bucket_scatter(int n, const int2* __restrict__ list, const int* __restrict__ pred,
+                                 const int64_t* __restrict__ col_ind, int64_t npat, const int* __restrict__ off,
+                                 int* __restrict__ fill, int2* __restrict__ out, int* __restrict__ pred_full) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int2 jt = list[i];
+  const int64_t key = pred[i] * npat + (col_ind[jt.x] - 1);
+  out[off[key] + atomicAdd(fill + key, 1)] = jt;
+  if (pred_full) pred_full[jt.x] = pred[i];
+}
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// One 16-pair block of the cached-tree check on the matrix cores.  Lane l evaluates
+// f = K(x_i, y_q) w_q for pair i = l % 16 and node slot l / 16 of a 4-node step; the FP64
+// MFMA 16x16x4 multiplies the 16x4 block of f by the 4x16 block of basis values P_j(y_q)
+// (j = l % 16 + 16 ct) and accumulates the moments of 16 pairs x 16 basis functions;
+// accumulator element r of lane l is pair l / 16 + 4 r, basis function l % 16 + 16 ct.
+template <int N, bool HELM>
+__device__ __forceinline__ void tile_moments(dbl4 (&T)[HELM ? 2 : 1][((N + 1) * (N + 2) / 2 + 15) / 16], int tri,
+                                             int kq, int nn, const double* __restrict__ cp,
+                                             const double* __restrict__ ptab, const double* tg, bool row_ok,
+                                             const KPar& kp) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  constexpr int CT = (NP + 15) / 16;
+  const int lane = threadIdx.x, ia = lane & 15, ka = lane >> 4;
+#pragma unroll 1
+  for (int q0 = 0; q0 < kq; q0 += 4) {
+    const int q = q0 + ka;
+    double fr = 0.0, fi = 0.0, bv[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) bv[ct] = 0.0;
+    if (q < kq) {
+      const int node = tri * kq + q;
+      double g[12];
+      g[0] = cp[node];
+      g[1] = cp[nn + node];
+      g[2] = cp[2 * nn + node];
+      g[9] = cp[3 * nn + node];
+      g[10] = cp[4 * nn + node];
+      g[11] = cp[5 * nn + node];
+      const double w = cp[6 * nn + node];
+      double vr, vi;
+      kern_eval<HELM>(tg, g, kp, vr, vi);
+      if (row_ok) {
+        fr = vr * w;
+        fi = vi * w;
+      }
+#pragma unroll
+      for (int ct = 0; ct < CT; ct++) {
+        const int j = ia + 16 * ct;
+        if (j < NP) bv[ct] = ptab[(size_t)node * NP + j];
+      }
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+      T[0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, bv[ct], T[0][ct], 0, 0, 0);
+      if (HELM) T[HELM ? 1 : 0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fi, bv[ct], T[HELM ? 1 : 0][ct], 0, 0, 0);
+    }
+  }
+}
+
+// per accumulator row r of this lane: max over basis functions of |A - B| (complex
+// modulus), reduced over the 16 lanes that share the row
+template <int NPART, int CT>
+__device__ __forceinline__ void tile_err(const dbl4 (&A)[NPART][CT], const dbl4 (&B)[NPART][CT], double* e) {
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    double m = 0.0;
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+      const double dr = A[0][ct][r] - B[0][ct][r];
+      double v = dr * dr;
+      if (NPART == 2) {
+        const double di = A[NPART - 1][ct][r] - B[NPART - 1][ct][r];
+        v += di * di;
+      }
+      v = sqrt(v);
+      m = v > m || v != v ? v : m;
+    }
+#pragma unroll
+    for (int off = 8; off >= 1; off >>= 1) {
+      const double o = __shfl_xor(m, off, 64);
+      m = o > m || o != o ? o : m;
+    }
+    e[r] = m;
+  }
+}
+
+// Cached-tree check for pairs whose predicted refinement tree stays inside depth DEPTH
+// (DEPTH 1: the root is not refined; DEPTH 2: only the root is refined).  The tests are
+// the adaptive integrator's own (triaintrouts.f:1435-1448) on the same triangles, so an
+// accepted pair gets the adaptive result; a pair that fails is appended to fb_list and
+// integrated by k_adaptive.
+template <int N, bool HELM, int DEPTH>
+__global__ void __launch_bounds__(64) k_shallow(int npairs, const int2* __restrict__ list, int kq,
+                                                const int64_t* __restrict__ col_ind, int64_t ndtarg,
+                                                const double* __restrict__ targs, const int64_t* __restrict__ iquad,
+                                                const double* __restrict__ epsp_patch,
+                                                const double* __restrict__ umat, KPar kp,
+                                                const double* __restrict__ cache, const double* __restrict__ ptab,
+                                                int* __restrict__ nref_out, int2* __restrict__ fb_list,
+                                                int* __restrict__ counts, double* __restrict__ wnear) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  constexpr int CT = (NP + 15) / 16;
+  constexpr int NPART = HELM ? 2 : 1;
+  constexpr int RW = NPART * CT * 16;  // staged result row: [part][padded j]
+  __shared__ double res[16 * RW];
+  __shared__ int code[16];
+  const int lane = threadIdx.x, ia = lane & 15, ka = lane >> 4;
+  const int base = blockIdx.x * 16;
+  const int nn = NCACHE_TRI * kq;
+  const bool row_ok = base + ia < npairs;
+  const int2 jta = list[row_ok ? base + ia : base];
+  const int64_t ipa = col_ind[jta.x] - 1;
+  const double* tp = targs + ndtarg * (int64_t)jta.y;
+  const double tg[3] = {tp[0], tp[1], tp[2]};
+  const double* cp = cache + (size_t)ipa * 7 * nn;
+  const dbl4 z4 = {0.0, 0.0, 0.0, 0.0};
+  dbl4 sumM[NPART][CT], T1[NPART][CT];
+#pragma unroll
+  for (int a = 0; a < NPART; a++)
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) sumM[a][ct] = z4;
+  double cmax[4] = {0.0, 0.0, 0.0, 0.0};
+  dbl4 sumS[NPART][CT];
+  if (DEPTH == 1) {
+#pragma unroll 1
+    for (int c = 1; c <= 4; c++) tile_moments<N, HELM>(sumM, c, kq, nn, cp, ptab, tg, row_ok, kp);
+  } else {
+    dbl4 T2[NPART][CT];
+#pragma unroll
+    for (int a = 0; a < NPART; a++)
+#pragma unroll
+      for (int ct = 0; ct < CT; ct++) sumS[a][ct] = z4;
+#pragma unroll 1
+    for (int c = 1; c <= 4; c++) {
+#pragma unroll
+      for (int a = 0; a < NPART; a++)
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) {
+          T1[a][ct] = z4;
+          T2[a][ct] = z4;
+        }
+      tile_moments<N, HELM>(T1, c, kq, nn, cp, ptab, tg, row_ok, kp);
+#pragma unroll 1
+      for (int g = 0; g < 4; g++) tile_moments<N, HELM>(T2, 5 + 4 * (c - 1) + g, kq, nn, cp, ptab, tg, row_ok, kp);
+      double e[4];
+      tile_err<NPART, CT>(T2, T1, e);
+#pragma unroll
+      for (int r = 0; r < 4; r++) cmax[r] = e[r] > cmax[r] || e[r] != e[r] ? e[r] : cmax[r];
+#pragma unroll
+      for (int a = 0; a < NPART; a++)
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) {
+          sumM[a][ct] += T1[a][ct];
+          sumS[a][ct] += T2[a][ct];
+        }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NPART; a++)
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) T1[a][ct] = z4;
+  tile_moments<N, HELM>(T1, 0, kq, nn, cp, ptab, tg, row_ok, kp);
+  double e0[4];
+  tile_err<NPART, CT>(sumM, T1, e0);
+  // decide per accumulator row and stage the accepted moments
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int i = ka + 4 * r;
+    int cd = -1;  // -1: padding row, 0: send back, 1: accepted at depth 1, 2: depth 2
+    if (base + i < npairs) {
+      const double eps = epsp_patch[col_ind[list[base + i].x] - 1];
+      // refine iff err > eps, as in the adaptive test
+      cd = !(e0[r] > eps) ? 1 : (DEPTH == 2 && !(cmax[r] > eps) ? 2 : 0);
+    }
+    if (ia == 0) code[i] = cd;
+#pragma unroll
+    for (int a = 0; a < NPART; a++)
+#pragma unroll
+      for (int ct = 0; ct < CT; ct++)
+        res[i * RW + a * CT * 16 + ct * 16 + ia] = (DEPTH == 2 && cd == 2) ? sumS[a][ct][r] : sumM[a][ct][r];
+  }
+  __syncthreads();
+  // weights = umat^T * moments for the accepted pairs
+  constexpr int NOUT = NPART * NP;
+  for (int o = lane; o < 16 * NOUT; o += 64) {
+    const int i = o / NOUT, rem = o - i * NOUT;
+    if (code[i] <= 0) continue;
+    const int j = rem / NPART, a = rem - j * NPART;
+    const double* m = res + i * RW + a * CT * 16;
+    double sacc = 0.0;
+    for (int l = 0; l < NP; l++) sacc = fma(umat[l + j * NP], m[l], sacc);
+    const int64_t q0 = iquad[list[base + i].x] - 1;
+    if (HELM) wnear[2 * (q0 + j) + a] = sacc;
+    else wnear[q0 + j] = sacc;
+  }
+  if (lane < 16 && code[lane] >= 0) {
+    const int2 jt = list[base + lane];
+    if (code[lane] == 0) {
+      fb_list[atomicAdd(counts + 4, 1)] = jt;
+    } else {
+      atomicAdd(counts + 4 + code[lane], 1);
+      if (nref_out) nref_out[jt.x] = code[lane] == 1 ? 1 : 5;
+    }
   }
 }
 
@@ -557,36 +919,38 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   double *d_umat = dalloc<double>(NP * NP), *d_rvw = dalloc<double>(NP), *d_xg = dalloc<double>(3 * kq),
          *d_fuv = dalloc<double>(2 * nf), *d_Pq = dalloc<double>((size_t)nf * NP),
          *d_Bq = dalloc<double>((size_t)nf * NP);
-  double *d_cms = dalloc<double>(3 * npat), *d_rads = dalloc<double>(npat), *d_epsp = dalloc<double>(npat);
+  double *d_cms = dalloc<double>(3 * npat), *d_rads = dalloc<double>(npat), *d_epsp = dalloc<double>(npat),
+         *d_area = dalloc<double>(npat), *d_rvuv = dalloc<double>(2 * NP);
   int *d_cnt = dalloc<int>(npat), *d_off = dalloc<int>(npat + 1), *d_fill = dalloc<int>(npat);
-  int *d_cls = dalloc<int>(A.nnz), *d_counts = dalloc<int>(4);
+  int *d_cls = dalloc<int>(A.nnz), *d_counts = dalloc<int>(8);
   int2 *d_adap = dalloc<int2>(A.nnz), *d_self = dalloc<int2>(A.nnz), *d_fixed = dalloc<int2>(A.nnz);
   int* d_nref = A.want_nref ? dalloc<int>(A.nnz) : nullptr;
   if (!d_umat || !d_rvw || !d_xg || !d_fuv || !d_Pq || !d_Bq || !d_cms || !d_rads || !d_epsp || !d_cnt ||
-      !d_off || !d_fill || !d_cls || !d_counts || !d_adap || !d_self || !d_fixed) {
+      !d_off || !d_fill || !d_cls || !d_counts || !d_adap || !d_self || !d_fixed || !d_area || !d_rvuv) {
     fprintf(stderr, "fmm3dbie_amd: device allocation failed in getnearquad\n");
     return -2;
   }
   HCHK(hipMemcpyAsync(d_umat, umat.data(), sizeof(double) * NP * NP, hipMemcpyHostToDevice, s));
   HCHK(hipMemcpyAsync(d_rvw, rvw.data(), sizeof(double) * NP, hipMemcpyHostToDevice, s));
+  HCHK(hipMemcpyAsync(d_rvuv, rvuv.data(), sizeof(double) * 2 * NP, hipMemcpyHostToDevice, s));
   HCHK(hipMemcpyAsync(d_xg, xg.data(), sizeof(double) * 3 * kq, hipMemcpyHostToDevice, s));
   HCHK(hipMemcpyAsync(d_fuv, fuv.data(), sizeof(double) * 2 * nf, hipMemcpyHostToDevice, s));
   HCHK(hipMemcpyAsync(d_Pq, Pq.data(), sizeof(double) * nf * NP, hipMemcpyHostToDevice, s));
   HCHK(hipMemcpyAsync(d_Bq, Bq.data(), sizeof(double) * nf * NP, hipMemcpyHostToDevice, s));
   HCHK(hipMemsetAsync(d_cnt, 0, sizeof(int) * npat, s));
   HCHK(hipMemsetAsync(d_fill, 0, sizeof(int) * npat, s));
-  HCHK(hipMemsetAsync(d_counts, 0, sizeof(int) * 4, s));
+  HCHK(hipMemsetAsync(d_counts, 0, sizeof(int) * 8, s));
   KPar kp;
   kp.zkr = A.zk[0]; kp.zki = A.zk[1];
   kp.ar = A.alpha[0]; kp.ai = A.alpha[1];
   kp.br = A.beta[0]; kp.bi = A.beta[1];
 
-  hipEvent_t ev[6];
+  hipEvent_t ev[10];
   for (auto& e : ev) HCHK(hipEventCreate(&e));
   HCHK(hipEventRecord(ev[0], s));
   const int tb = 256;
   k_patch_prep<N><<<(npat + tb - 1) / tb, tb, 0, s>>>(npat, A.ixyzs, A.srccoefs, A.srcvals, d_rvw, eps_adap,
-                                                      d_cms, d_rads, d_epsp);
+                                                      d_cms, d_rads, d_epsp, d_area);
   k_classify<<<(A.ntarg + tb - 1) / tb, tb, 0, s>>>(A.ntarg, A.ndtarg, A.targs, A.ipatch_id, A.row_ptr,
                                                     A.col_ind, d_cms, d_rads, A.rfac0, d_cnt, d_adap,
                                                     d_counts + 0, d_self, d_counts + 1, d_cls);
@@ -604,6 +968,40 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
                                                 A.iquad, d_off, d_fixed, kp, A.wnear);
   HCHK(hipGetLastError());
   HCHK(hipEventRecord(ev[2], s));
+  // node tables of the cached tree depths 0..2 (patch independent)
+  const int ncn = NCACHE_TRI * kq;
+  std::vector<double> tuv(2 * ncn), twq(ncn), ptab((size_t)ncn * NP);
+  {
+    struct Tri { double x0, y0, dx, dy; };
+    std::vector<Tri> tris(NCACHE_TRI);
+    tris[0] = {0.0, 0.0, 1.0, 1.0};
+    auto kids = [](Tri t, int c) -> Tri {
+      const double hx = t.dx / 2, hy = t.dy / 2;
+      const Tri k[4] = {{t.x0, t.y0, hx, hy}, {t.x0 + hx, t.y0, hx, hy}, {t.x0, t.y0 + hy, hx, hy},
+                        {t.x0 + hx, t.y0 + hy, -hx, -hy}};
+      return k[c];
+    };
+    for (int c = 0; c < 4; c++) tris[1 + c] = kids(tris[0], c);
+    for (int t = 1; t <= 4; t++)
+      for (int c = 0; c < 4; c++) tris[5 + 4 * (t - 1) + c] = kids(tris[t], c);
+    for (int t = 0; t < NCACHE_TRI; t++) {
+      const double da = t == 0 ? 1.0 : (t <= 4 ? 0.25 : 0.0625);
+      for (int q = 0; q < kq; q++) {
+        const int n = t * kq + q;
+        tuv[2 * n] = tris[t].x0 + tris[t].dx * xq[2 * q];
+        tuv[2 * n + 1] = tris[t].y0 + tris[t].dy * xq[2 * q + 1];
+        twq[n] = da * xw[q];
+        fmmhost::koorn_pols(tuv[2 * n], tuv[2 * n + 1], N, pols.data());
+        for (int l = 0; l < NP; l++) ptab[(size_t)n * NP + l] = pols[l];
+      }
+    }
+  }
+  double *d_tuv = dalloc<double>(2 * ncn), *d_twq = dalloc<double>(ncn), *d_ptab = dalloc<double>((size_t)ncn * NP);
+  double* d_cache = nadap > 0 ? dalloc<double>((size_t)npat * 7 * ncn) : nullptr;
+  if (!d_tuv || !d_twq || !d_ptab || (nadap > 0 && !d_cache)) return -2;
+  HCHK(hipMemcpyAsync(d_tuv, tuv.data(), sizeof(double) * 2 * ncn, hipMemcpyHostToDevice, s));
+  HCHK(hipMemcpyAsync(d_twq, twq.data(), sizeof(double) * ncn, hipMemcpyHostToDevice, s));
+  HCHK(hipMemcpyAsync(d_ptab, ptab.data(), sizeof(double) * ncn * NP, hipMemcpyHostToDevice, s));
   // adaptive
   const int nwaves = A.adaptive_waves > 0 ? A.adaptive_waves : 4096;
   constexpr int ES = 6 + 2 * NP;
@@ -611,12 +1009,77 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   if (!d_stack) return -2;
   int* d_over = d_counts + 2;
   int* d_ctr = d_counts + 3;
-  if (nadap > 0) {
-    const int g = nadap < nwaves ? nadap : nwaves;
-    k_adaptive<N, HELM><<<g, 64, 0, s>>>(nadap, d_adap, d_ctr, kq, d_xg, A.col_ind, A.ixyzs, A.srccoefs,
-                                         A.ndtarg, A.targs, A.iquad, d_epsp, d_umat, kp, d_stack, d_nref, d_over,
-                                         A.wnear);
+  // learned path buffers
+  const bool learned = A.mlp_params != nullptr && N <= 6 && nadap > 0;
+  const bool want_feat = (learned || A.feat_out) && nadap > 0;
+  float* d_X = want_feat ? dalloc<float>((size_t)nadap * 8) : nullptr;
+  int *d_pred = nullptr, *d_boff = nullptr, *d_bcnt = nullptr, *d_pfull = nullptr;
+  int2 *d_sorted = nullptr, *d_fb = nullptr;
+  if (want_feat && !d_X) return -2;
+  if (learned) {
+    d_pred = dalloc<int>(nadap);
+    d_bcnt = dalloc<int>(MLP_OUT * npat);
+    d_boff = dalloc<int>(MLP_OUT * npat + 1);
+    d_sorted = dalloc<int2>(nadap);
+    d_fb = dalloc<int2>(nadap);
+    if (A.pred_host) d_pfull = dalloc<int>(A.nnz);
+    if (!d_pred || !d_bcnt || !d_boff || !d_sorted || !d_fb || (A.pred_host && !d_pfull)) return -2;
+  }
+  HCHK(hipEventRecord(ev[5], s));
+  int nb[3] = {0, 0, 0};  // pairs predicted depth 1, depth 2, deeper
+  if (want_feat) {
+    const double kabs = HELM ? sqrt(A.zk[0] * A.zk[0] + A.zk[1] * A.zk[1]) : 0.0;
+    k_features<N><<<(nadap + tb - 1) / tb, tb, 0, s>>>(nadap, d_adap, A.col_ind, A.ixyzs, A.srcvals, A.ndtarg,
+                                                       A.targs, d_cms, d_rads, d_area, d_epsp, d_rvuv, kabs, d_X,
+                                                       A.feat_out);
     HCHK(hipGetLastError());
+  }
+  if (learned) {
+    if (mlp_predict_device(A.mlp_params, d_X, nadap, d_pred, nullptr, s) != 0) return -1;
+    HCHK(hipMemsetAsync(d_bcnt, 0, sizeof(int) * MLP_OUT * npat, s));
+    if (d_pfull) HCHK(hipMemsetAsync(d_pfull, 0xff, sizeof(int) * A.nnz, s));
+    k_bucket_count<<<(nadap + tb - 1) / tb, tb, 0, s>>>(nadap, d_adap, d_pred, A.col_ind, npat, d_bcnt);
+    k_scan<<<1, 1024, 0, s>>>(MLP_OUT * npat, d_bcnt, d_boff);
+    HCHK(hipMemsetAsync(d_bcnt, 0, sizeof(int) * MLP_OUT * npat, s));
+    k_bucket_scatter<<<(nadap + tb - 1) / tb, tb, 0, s>>>(nadap, d_adap, d_pred, A.col_ind, npat, d_boff, d_bcnt,
+                                                          d_sorted, d_pfull);
+    int b1 = 0, b2 = 0;
+    HCHK(hipMemcpyAsync(&b1, d_boff + npat, sizeof(int), hipMemcpyDeviceToHost, s));
+    HCHK(hipMemcpyAsync(&b2, d_boff + 2 * npat, sizeof(int), hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    nb[0] = b1;
+    nb[1] = b2 - b1;
+    nb[2] = nadap - b2;
+  }
+  HCHK(hipEventRecord(ev[6], s));
+  if (nadap > 0) {
+    k_tree_cache<N><<<npat, 256, 0, s>>>(npat, kq, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs, d_cache);
+    HCHK(hipEventRecord(ev[8], s));
+    if (learned) {
+      if (nb[0] > 0)
+        k_shallow<N, HELM, 1><<<(nb[0] + 15) / 16, 64, 0, s>>>(nb[0], d_sorted, kq, A.col_ind, A.ndtarg, A.targs,
+                                                               A.iquad, d_epsp, d_umat, kp, d_cache, d_ptab, d_nref,
+                                                               d_fb, d_counts, A.wnear);
+      if (nb[1] > 0)
+        k_shallow<N, HELM, 2><<<(nb[1] + 15) / 16, 64, 0, s>>>(nb[1], d_sorted + nb[0], kq, A.col_ind, A.ndtarg,
+                                                               A.targs, A.iquad, d_epsp, d_umat, kp, d_cache,
+                                                               d_ptab, d_nref, d_fb, d_counts, A.wnear);
+      HCHK(hipGetLastError());
+    }
+    HCHK(hipEventRecord(ev[7], s));
+    const int g = nadap < nwaves ? nadap : nwaves;
+    if (learned)
+      k_adaptive<N, HELM><<<g, 64, 0, s>>>(nb[2], d_sorted + nb[0] + nb[1], 1, d_fb, d_counts + 4, d_ctr, kq, d_xg,
+                                           A.col_ind, A.ixyzs, A.srccoefs, A.ndtarg, A.targs, A.iquad, d_epsp,
+                                           d_umat, kp, d_stack, d_nref, d_over, A.wnear, d_cache, d_ptab);
+    else
+      k_adaptive<N, HELM><<<g, 64, 0, s>>>(nadap, d_adap, 0, nullptr, nullptr, d_ctr, kq, d_xg, A.col_ind,
+                                           A.ixyzs, A.srccoefs, A.ndtarg, A.targs, A.iquad, d_epsp, d_umat, kp,
+                                           d_stack, d_nref, d_over, A.wnear, d_cache, d_ptab);
+    HCHK(hipGetLastError());
+  } else {
+    HCHK(hipEventRecord(ev[8], s));
+    HCHK(hipEventRecord(ev[7], s));
   }
   HCHK(hipEventRecord(ev[3], s));
   if (nself > 0) {
@@ -630,8 +1093,11 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     HCHK(hipGetLastError());
   }
   HCHK(hipEventRecord(ev[4], s));
-  HCHK(hipMemcpyAsync(counts, d_counts, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+  int counts8[8];
+  HCHK(hipMemcpyAsync(counts8, d_counts, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
   HCHK(hipStreamSynchronize(s));
+  for (int i = 0; i < 4; i++) counts[i] = counts8[i];
+  if (d_pfull) hipMemcpy(A.pred_host, d_pfull, sizeof(int) * A.nnz, hipMemcpyDeviceToHost);
   if (st) {
     float ms;
     st->nself = nself;
@@ -642,7 +1108,14 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     st->kq = kq;
     hipEventElapsedTime(&ms, ev[0], ev[1]); st->ms_prep = ms;
     hipEventElapsedTime(&ms, ev[1], ev[2]); st->ms_fixed = ms;
-    hipEventElapsedTime(&ms, ev[2], ev[3]); st->ms_adap = ms;
+    hipEventElapsedTime(&ms, ev[2], ev[5]); st->ms_tables = ms;
+    hipEventElapsedTime(&ms, ev[5], ev[6]); st->ms_select = ms;
+    hipEventElapsedTime(&ms, ev[6], ev[8]); st->ms_tree = ms;
+    hipEventElapsedTime(&ms, ev[8], ev[7]); st->ms_shallow = ms;
+    hipEventElapsedTime(&ms, ev[7], ev[3]); st->ms_adap = ms;
+    st->nshallow0 = counts8[5];
+    st->nshallow1 = counts8[6];
+    st->nfallback = counts8[4];
     hipEventElapsedTime(&ms, ev[3], ev[4]); st->ms_self = ms;
     st->nref_total = 0;
     if (d_nref && A.nref_host) {
@@ -652,7 +1125,8 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   }
   for (auto& e : ev) hipEventDestroy(e);
   void* frees[] = {d_umat, d_rvw, d_xg, d_fuv, d_Pq, d_Bq, d_cms, d_rads, d_epsp, d_cnt, d_off, d_fill,
-                   d_cls, d_counts, d_adap, d_self, d_fixed, d_nref, d_stack};
+                   d_cls, d_counts, d_adap, d_self, d_fixed, d_nref, d_stack, d_tuv, d_twq, d_ptab, d_cache,
+                   d_area, d_rvuv, d_X, d_pred, d_bcnt, d_boff, d_sorted, d_fb, d_pfull};
   for (void* p : frees)
     if (p) hipFree(p);
   return 0;

@@ -151,15 +151,40 @@ int fmm3dbie_last_error(void);
 
 /* getnearquad_*_comb_dir(_eval) on device buffers.  kpar6 = (k re, k im, alpha re,
  * alpha im, beta re, beta im); helm = 0 selects the Laplace kernel (real weights).
- * stats (optional, 12 doubles): nself, nadap, nfixed, overflow, nf, kq, and the
- * prep/fixed/adaptive/self kernel times in ms.  nref_host/class_host (optional, nnz
- * ints): refinements per pair and pair class (0 self, 1 adaptive, 2 fixed). */
+ * stats (optional, 20 doubles): nself, nadap, nfixed, overflow, nf, kq; times in ms of
+ * prep, fixed, adaptive, self, host tables, selector, tree cache, cached-tree kernels;
+ * pairs accepted at depth 1 and 2 by the cached-tree kernels, pairs sent back to the
+ * adaptive kernel; 3 unused.  nref_host/class_host (optional, nnz ints): refinements per
+ * pair and pair class (0 self, 1 adaptive, 2 fixed).
+ * mlp_params (optional, device, fmm3dbie_mlp_nparam() floats): learned refinement-depth
+ * selector.  The adaptive pairs (ggq-quads.f:382-386) are routed by predicted depth to
+ * the cached-tree kernels, which apply the adaptive test of triaintrouts.f:1435-1448 to
+ * the depth-0..2 triangles and send failures back to the adaptive kernel; the rest run
+ * longest-predicted first.  Orders 1..6; ignored for higher orders.  feat_out (optional,
+ * device, nnz x 8 floats): selector features of the adaptive pairs.  pred_host
+ * (optional, nnz ints): predicted class (-1 for non-adaptive pairs). */
 int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatches, const int64_t *ixyzs,
                           const double *srccoefs, const double *srcvals, int64_t ndtarg, int64_t ntarg,
                           const double *targs, const int64_t *ipatch_id, const double *uvs_targ, double eps,
                           const double *kpar6, int64_t nnz, const int64_t *row_ptr, const int64_t *col_ind,
                           const int64_t *iquad, double rfac0, double *wnear, int self_n, double self_tau_panel,
-                          void *stream, double *stats, int *nref_host, int *class_host);
+                          void *stream, double *stats, int *nref_host, int *class_host, const float *mlp_params,
+                          float *feat_out, int *pred_host);
+
+/* ---- learned refinement-depth selector (no reference counterpart) --------------------
+ * MLP 8 -> 64 -> 64 -> 6 (ReLU, softmax), FP32, parameters packed as W1[64][8] b1[64]
+ * W2[64][64] b2[64] W3[6][64] b3[6]; X is n x 8 row-major, y class ids 0..5 (class =
+ * min(5, (nref - 1) / 4), the number of refined triangles of the adaptive tree). */
+int fmm3dbie_mlp_nparam(void);
+/* mean softmax cross-entropy over n samples and its gradient (device pointers) */
+int fmm3dbie_mlp_grad(const float *params, const float *X, const int *y, int64_t n, float *grad, float *loss,
+                      void *stream);
+/* minibatch Adam (batch a multiple of 64), reshuffled every epoch from seed;
+ * loss_hist (host, epochs floats, optional): last minibatch loss of each epoch */
+int fmm3dbie_mlp_train(float *params, const float *X, const int *y, int64_t n, int epochs, int batch, float lr,
+                       unsigned seed, float *loss_hist, void *stream);
+/* argmax class (and optionally the n x 6 logits) */
+int fmm3dbie_mlp_predict(const float *params, const float *X, int64_t n, int *cls, float *logits, void *stream);
 
 /* Device-resident apply plan for lpcomp_*_addsub (host bookkeeping arrays; wnear on the
  * host or, with wnear_on_device = 1, already in HBM). */
