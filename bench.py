@@ -60,7 +60,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-learned", action="store_true", help="skip the torus learned+mixed solve")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-plane-wave batched solve")
+    ap.add_argument("--batched-sequential", type=int, default=8,
+                    help="FP64 single-RHS solves timed for the batched comparison")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device read + 1 GiB write (torch.neg) for HBM counter scaling")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"read_bytes": a.numel() * 4, "write_bytes": b.numel() * 4,
+                                               "launches": 3}}))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
               file=sys.stderr)
@@ -194,7 +210,147 @@ def main():
         plan.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, helm, pars, eps, P, args.cpu_seconds)
+    if args.config == 1 and not args.no_solve:
+        if not args.no_learned:
+            out["learned_mixed_solve"] = learned_mixed_solve(dev)
+        if not args.no_batched:
+            out["batched_solve"] = batched_solve(dev, args.batched_sequential)
     print(json.dumps(out))
+
+
+def _exterior_field(g, wts, x, targ, pars, zk):
+    """(alpha S + beta D) x at off-surface targets by the smooth rule (targets far away)."""
+    d = targ[:, None, :] - g["srcvals"][None, :, :3]
+    r = np.linalg.norm(d, axis=-1)
+    dn = (d * g["srcvals"][None, :, 9:12]).sum(-1)
+    ker = (pars[2] * dn * (1 - 1j * zk * r) / r ** 3 + pars[1] / r) * np.exp(1j * zk * r) / (4 * np.pi)
+    return (np.atleast_2d(x) * wts[None, :]) @ ker.T
+
+
+def _timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = fn()
+    torch.cuda.synchronize()
+    return r, time.perf_counter() - t0
+
+
+def learned_mixed_solve(dev):
+    """Wavy torus 48x16 cells, order 6, zk=3, interior point source at (2, 0, 0.05):
+    plain FP64 GPU form (adaptive quadrature, FP64 GMRES) against learned + mixed (selector
+    routed quadrature, FP32 inner GMRES with FP64 iterative refinement), both to 1e-6."""
+    import torch
+    from fmm3dbie_amd import geometry as G, layer, problem, selector as S
+    zk = 3.0
+    pars = np.array([zk, 1j * zk, 1.0])
+    g = G.get_wavy_torus(48, 16, 6)
+    P = problem.setup(g, 1e-6, True, pars)
+    x0 = np.array([2.0, 0.0, 0.05])
+    rhs = torch.as_tensor(problem.point_source(g["srcvals"][:, :3], x0, zk), device=dev)
+    wts = torch.as_tensor(P["wts"], device=dev)
+    nq = layer.NearQuadDevice(g, True, pars, 1e-6, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], device=dev)
+    # labels and features from one plain run, GPU training on 2e5 sampled pairs (seed 3)
+    X, y, idx = S.collect(nq)
+    params, info = S.train(X, y, n_train=200000, epochs=30, seed=3, device=dev)
+
+    def plain():
+        w = nq.run()
+        plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"], P["ixyzso"],
+                               P["srcover"], P["whtsover"])
+        t0 = time.perf_counter()
+        x, nit, _, rres, _ = layer.gmres(plan, rhs, 0.5, wts, numit=300, eps_gmres=1e-6)
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t0
+        plan.close()
+        return x, int(nit[0]), float(rres[0]), ts
+
+    def learned():
+        w = nq.run(selector=params)
+        plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"], P["ixyzso"],
+                               P["srcover"], P["whtsover"])
+        plan.tune(32, 1)
+        t0 = time.perf_counter()
+        x, outer, inner, rr, n32, n64 = layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-4, tol=1e-6)
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t0
+        plan.close()
+        return x, outer, inner, float(rr[0]), ts
+
+    plain()
+    (x64, nit, rr64, ts64), tt64 = _timed(plain)
+    nq.run()
+    q64 = {k: nq.stats[k] for k in ("ms_tree", "ms_adap", "ms_fixed", "ms_self")}
+    learned()
+    (xl, outer, inner, rrl, tsl), ttl = _timed(learned)
+    ql = {k: nq.stats[k] for k in ("ms_select", "ms_tree", "ms_shallow", "ms_adap", "ms_fixed", "ms_self",
+                                   "nshallow0", "nshallow1", "nfallback")}
+    targ = np.array([[4.0, 0.0, 1.0], [0.0, 4.0, -1.0], [-3.5, 1.0, 2.0], [0.5, -3.8, 0.5], [0.0, 0.0, 3.0],
+                     [2.0, 2.0, 2.5], [-2.0, 2.5, -2.0], [3.0, -3.0, 0.0], [0.0, 0.0, -3.5], [-4.0, -1.0, 0.0]])
+    exact = problem.point_source(targ, x0, zk)
+    e64 = float(np.abs(_exterior_field(g, P["wts"], x64.cpu().numpy(), targ, pars, zk) - exact).max()
+                / np.abs(exact).max())
+    el = float(np.abs(_exterior_field(g, P["wts"], xl.cpu().numpy(), targ, pars, zk) - exact).max()
+               / np.abs(exact).max())
+    xd = (xl - x64).cpu().numpy()
+    rel = float(np.sqrt((np.abs(xd) ** 2 * P["wts"]).sum() / (np.abs(x64.cpu().numpy()) ** 2 * P["wts"]).sum()))
+    return {"workload": "helmholtz_zk3_wavytorus_48x16_o6_pointsource", "npts": len(g["srcvals"]), "nnz": P["nnz"],
+            "plain_fp64": {"setup_plus_solve_s": tt64, "solve_s": ts64, "iters": nit, "rres": rr64,
+                           "quad_ms": q64, "field_err": e64},
+            "learned_mixed": {"setup_plus_solve_s": ttl, "solve_s": tsl, "ir_outer": outer, "ir_inner_iters": inner,
+                              "rres": rrl, "quad_ms": ql, "field_err": el, "solution_rel_vs_fp64": rel},
+            "speedup_solve": ts64 / tsl, "speedup_setup_plus_solve": tt64 / ttl,
+            "selector": {"train_pairs": info["n_train"], "heldout_pairs": info["n_heldout"],
+                         "acc_train": info["acc_train"], "acc_heldout": info["acc_heldout"],
+                         "train_ms": info["ms_train"], "loss_first_last": [info["loss"][0], info["loss"][-1]]}}
+
+
+def batched_solve(dev, nseq):
+    """Sphere 12x12 per face, order 4, zk=5, 64 plane waves (directions from seed 2),
+    sound-soft data f = -u_inc: 64 batched systems with the autotuned FP32 far sum and
+    FP64 iterative refinement against FP64 GMRES one right-hand side at a time (timed on
+    nseq of them and scaled to 64)."""
+    import torch
+    from fmm3dbie_amd import geometry as G, layer, problem
+    zk = 5.0
+    pars = np.array([zk, 1j * zk, 1.0])
+    g = G.get_sphere_npat(1.0, 12, (0, 0, 0), 4)
+    P = problem.setup(g, 1e-6, True, pars)
+    dirs = problem.plane_wave_dirs(64, 2)
+    rhs = torch.as_tensor(-problem.plane_waves(g["srcvals"][:, :3], dirs, zk), device=dev)
+    wts = torch.as_tensor(P["wts"], device=dev)
+    nq = layer.NearQuadDevice(g, True, pars, 1e-6, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], device=dev)
+    w = nq.run()
+    plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"], P["ixyzso"],
+                           P["srcover"], P["whtsover"])
+    layer.gmres(plan, rhs[0], 0.5, wts, numit=300, eps_gmres=1e-6)
+    xs, iters = [], []
+    _, tseq = _timed(lambda: [xs.append(layer.gmres(plan, rhs[r], 0.5, wts, numit=300, eps_gmres=1e-6))
+                              for r in range(nseq)])
+    iters = [int(x[1][0]) for x in xs]
+    x64 = torch.stack([x[0] for x in xs]).cpu().numpy()
+    tune, ttune = _timed(lambda: plan.tune(32, 64))
+    best = min(tune, key=lambda t: t[2])
+    layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-4, tol=1e-6)
+    (xb, outer, inner, rr, n32, n64), tb = _timed(
+        lambda: layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-4, tol=1e-6))
+    xb = xb.cpu().numpy()
+    rel = max(float(np.sqrt((np.abs(xb[r] - x64[r]) ** 2 * P["wts"]).sum() / (np.abs(x64[r]) ** 2 * P["wts"]).sum()))
+              for r in range(nseq))
+    targ = 3.0 * problem.plane_wave_dirs(10, 0)
+    exact = problem.mie_sound_soft(targ, dirs[:8], zk)
+    u = _exterior_field(g, P["wts"], xb[:8], targ, pars, zk)
+    emie = float(np.abs(u - exact).max() / np.abs(exact).max())
+    tm = np.zeros(4)
+    plan.apply(rhs.to(torch.complex64), 32, 1, times=tm)
+    plan.close()
+    return {"workload": "helmholtz_zk5_sphere_12x12_o4_64planewaves", "npts": len(g["srcvals"]), "nrhs": 64,
+            "fp64_sequential_s_per_rhs": tseq / nseq, "fp64_sequential_s_64_scaled": tseq / nseq * 64,
+            "fp64_sequential_timed_rhs": nseq, "fp64_iters": iters,
+            "batched_mixed_s": tb, "ir_outer": outer, "ir_inner_iters": inner, "ir_rres_max": float(np.max(rr)),
+            "speedup_vs_fp64_sequential": tseq / nseq * 64 / tb, "solution_rel_vs_fp64_max": rel,
+            "mie_field_err_8rhs": emie, "tuned_far": {"group": best[0], "wg_target": best[1], "ms": best[2]},
+            "tune_s": ttune, "apply_fp32_64rhs_ms": {"oversample": tm[0], "far": tm[1], "reduce_csr": tm[2]}}
 
 
 def cpu_baseline(g, helm, pars, eps, P, budget_s):

@@ -34,6 +34,7 @@ SIGS = {
     "fmm3dbie_apply_plan_apply": ([vp, i32, i32, vp, vp, i32, vp, vp], i32),
     "fmm3dbie_apply_plan_info": ([vp, vp], i32),
     "fmm3dbie_apply_plan_destroy": ([vp], None),
+    "fmm3dbie_apply_plan_tune": ([vp, i32, i32, vp, vp, i32], i32),
     "fmm3dbie_gmres_dev": ([vp, i32, i32, vp, vp, vp, vp, i32, i32, f64, vp, vp, vp, vp, vp], i32),
     "fmm3dbie_gmres_ir_dev": ([vp, i32, vp, vp, vp, vp, i32, i32, f64, f64, i32, vp, vp, vp, vp, vp], i32),
 }

@@ -115,6 +115,11 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
   __shared__ T lx[TPB], ly[TPB], lz[TPB], lnx[TPB], lny[TPB], lnz[TPB];
   __shared__ V2<T> lq[NRHS][TPB], lm[NRHS][TPB];
   __shared__ int lp[TPB];
+  // blockIdx.z selects a group of NRHS right-hand sides: the geometry and e^{ikr} of a
+  // pair are computed once per group
+  const int64_t rg = blockIdx.z;
+  qstr += rg * NRHS * ldstr;
+  mstr += rg * NRHS * ldstr;
   const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
   const bool act = i < ntarg;
   const T x = act ? tx[i] : (T)0, y = act ? ty[i] : (T)0, z = act ? tz[i] : (T)0;
@@ -172,7 +177,7 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
   }
   if (act) {
 #pragma unroll
-    for (int r = 0; r < NRHS; r++) part[(int64_t)blockIdx.y * ldpart + r * ntarg + i] = {accr[r], acci[r]};
+    for (int r = 0; r < NRHS; r++) part[(int64_t)blockIdx.y * ldpart + (rg * NRHS + r) * ntarg + i] = {accr[r], acci[r]};
   }
 }
 
@@ -287,6 +292,65 @@ __global__ void k_csr_gen(int64_t ntarg, const int64_t* __restrict__ row_ptr, co
     pot[r * ldpot + i].x += (T)sr;
     pot[r * ldpot + i].y += (T)si;
   }
+}
+
+// Far-sum reduction fused with the CSR near correction for all right-hand sides (uniform
+// patch size p): one wavefront per target row, lanes over the row's contiguous wnear
+// segment; every wnear element is loaded once per group of RB right-hand sides.
+//   pot[r][i] = (sum over source splits of part[r][i]) / (4 pi) + sum_e wnear_e sigma_r(src_e)
+template <typename T, typename TW, int RB>
+__global__ void k_reduce_csr(int64_t ntarg, int nrhs, int nsplit, const V2<T>* __restrict__ part, int64_t ldpart,
+                             const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ind,
+                             const int64_t* __restrict__ iquad, const int64_t* __restrict__ ixyzs, int p,
+                             const V2<TW>* __restrict__ wnear, const V2<T>* __restrict__ sigma, int64_t ldsig,
+                             V2<T>* __restrict__ pot, int64_t ldpot) {
+  const int64_t i = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (i >= ntarg) return;
+  const int64_t j0 = row_ptr[i] - 1, j1 = row_ptr[i + 1] - 1;
+  const int64_t q0 = j0 < j1 ? iquad[j0] - 1 : 0;
+  const int64_t ne = (j1 - j0) * (int64_t)p;
+  for (int rb = 0; rb < nrhs; rb += RB) {
+    double ar[RB], ai[RB];
+#pragma unroll
+    for (int r = 0; r < RB; r++) { ar[r] = 0.0; ai[r] = 0.0; }
+    for (int64_t e = lane; e < ne; e += 64) {
+      const int64_t jj = j0 + e / p;
+      const int64_t src = ixyzs[col_ind[jj] - 1] - 1 + e % p;
+      const V2<TW> w = wnear[q0 + e];
+#pragma unroll
+      for (int r = 0; r < RB; r++) {
+        if (rb + r < nrhs) {
+          const V2<T> sg = sigma[(rb + r) * ldsig + src];
+          ar[r] += (double)w.x * (double)sg.x - (double)w.y * (double)sg.y;
+          ai[r] += (double)w.x * (double)sg.y + (double)w.y * (double)sg.x;
+        }
+      }
+    }
+    double mr = 0.0, mi = 0.0;
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      const double sr = wave_sum(ar[r]), si = wave_sum(ai[r]);
+      if (lane == r) { mr = sr; mi = si; }
+    }
+    const int r = rb + lane;
+    if (lane < RB && r < nrhs) {
+      T fr = 0, fi = 0;
+      for (int sp = 0; sp < nsplit; sp++) {
+        const V2<T> v = part[(int64_t)sp * ldpart + r * ntarg + i];
+        fr += v.x;
+        fi += v.y;
+      }
+      pot[r * ldpot + i] = {(T)(fr * (T)OVER4PI + (T)mr), (T)(fi * (T)OVER4PI + (T)mi)};
+    }
+  }
+}
+
+// constant fill (tuning densities)
+template <typename T>
+__global__ void k_fill(int64_t n, T* __restrict__ a, T v) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
 }
 
 __global__ void k_split_src(int64_t n, const double* __restrict__ s12, double* __restrict__ x, double* __restrict__ y,
@@ -503,8 +567,8 @@ static int ensure_ws(ApplyPlan* P, int nrhs) {
   P->m32 = dal<float>(2 * NO * nrhs);
   P->sig64c = dal<double>(2 * (size_t)P->npts);
   P->pot64c = dal<double>(2 * (size_t)P->npts);
-  // partial far sums: far_split_cap splits x up to 8 batched columns x npts
-  const size_t npart = (size_t)P->far_split_cap * 8 * P->npts;
+  // partial far sums: nsplit x nrhs x npts with nsplit x nrhs <= far_split_cap x 64
+  const size_t npart = (size_t)P->far_split_cap * 64 * P->npts;
   P->part64 = dal<double>(2 * npart);
   P->part32 = dal<float>(2 * npart);
   if (!P->q64 || !P->m64 || !P->q32 || !P->m32 || !P->sig64c || !P->pot64c || !P->part64 || !P->part32) return -2;
@@ -521,9 +585,13 @@ void apply_plan_free_ws(ApplyPlan* P) {
   P->ws_cap = 0;
 }
 
+// Far sum of nrhs right-hand sides in groups of NRHS (grid z), then the reduction over
+// source splits fused with the CSR correction (uniform p) or followed by the general CSR
+// kernel.  The split count keeps the grid at >= wg_target workgroups, within the partial
+// buffer (far_split_cap x 64 x npts).
 template <typename T, int NRHS>
-static void launch_far(ApplyPlan* P, bool mask, const V2<T>* q, const V2<T>* m, int64_t ldstr, V2<T>* pot,
-                       int64_t ldpot, hipStream_t s) {
+static int launch_far(ApplyPlan* P, bool mask, int64_t wg_target, const V2<T>* q, const V2<T>* m, int64_t ldstr,
+                      int nrhs, V2<T>* part, int64_t* nsplit_out, hipStream_t s) {
   const T* const* src;
   const T* const* trg;
   if constexpr (sizeof(T) == 8) {
@@ -533,25 +601,20 @@ static void launch_far(ApplyPlan* P, bool mask, const V2<T>* q, const V2<T>* m, 
     src = (const T* const*)P->src32;
     trg = (const T* const*)P->trg32;
   }
-  if (P->npts == 0) return;
-  if (P->nptso == 0) {
-    for (int r = 0; r < NRHS; r++) (void)hipMemsetAsync(pot + r * ldpot, 0, sizeof(V2<T>) * P->npts, s);
-    return;
-  }
   const T kr = (T)(P->helm ? P->zk[0] : 0.0), ki = (T)(P->helm ? P->zk[1] : 0.0);
   const T th2 = (T)(P->thresh * P->thresh);
   const unsigned g = nblk(P->npts, TPB);
-  // split the sources so that the grid holds >= ~8 workgroups per CU
+  const int ngrp = nrhs / NRHS;
   const int64_t ntiles = (P->nptso + TPB - 1) / TPB;
-  int64_t nsplit = (P->far_wg_target + g - 1) / g;
-  if (nsplit < 1) nsplit = 1;
+  int64_t nsplit = (wg_target + (int64_t)g * ngrp - 1) / ((int64_t)g * ngrp);
+  const int64_t cap = (int64_t)P->far_split_cap * 64 / nrhs;
+  if (nsplit > cap) nsplit = cap;
   if (nsplit > ntiles) nsplit = ntiles;
-  if (nsplit > P->far_split_cap) nsplit = P->far_split_cap;
+  if (nsplit < 1) nsplit = 1;
   const int64_t chunk = ((ntiles + nsplit - 1) / nsplit) * TPB;
   nsplit = (P->nptso + chunk - 1) / chunk;
-  V2<T>* part = (V2<T>*)(sizeof(T) == 8 ? (void*)P->part64 : (void*)P->part32);
-  const int64_t ldpart = (int64_t)NRHS * P->npts;
-  dim3 grid(g, (unsigned)nsplit);
+  const int64_t ldpart = (int64_t)nrhs * P->npts;
+  dim3 grid(g, (unsigned)nsplit, (unsigned)ngrp);
   if (mask)
     k_farsum<T, true, NRHS><<<grid, TPB, 0, s>>>(P->npts, trg[0], trg[1], trg[2], P->nptso, src[0], src[1], src[2],
                                                  src[3], src[4], src[5], P->opatch, q, m, ldstr, P->row_ptr,
@@ -560,8 +623,69 @@ static void launch_far(ApplyPlan* P, bool mask, const V2<T>* q, const V2<T>* m, 
     k_farsum<T, false, NRHS><<<grid, TPB, 0, s>>>(P->npts, trg[0], trg[1], trg[2], P->nptso, src[0], src[1],
                                                   src[2], src[3], src[4], src[5], P->opatch, q, m, ldstr,
                                                   P->row_ptr, P->near_sorted, kr, ki, th2, chunk, part, ldpart);
-  k_farsum_reduce<T><<<dim3(nblk(P->npts, 256), NRHS), 256, 0, s>>>(P->npts, NRHS, (int)nsplit, part, ldpart, pot,
-                                                                     ldpot);
+  *nsplit_out = nsplit;
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename T>
+static int far_group(const ApplyPlan* P, int nrhs, int prec, int64_t* wg) {
+  const auto it = P->tuned.find(prec * 1000 + nrhs);
+  if (it != P->tuned.end()) {
+    *wg = it->second.second;
+    return it->second.first;
+  }
+  *wg = P->far_wg_target;
+  const int pref = sizeof(T) == 8 ? 4 : 8;
+  for (int gsz = pref; gsz > 1; gsz >>= 1)
+    if (nrhs % gsz == 0) return gsz;
+  return 1;
+}
+
+template <typename T>
+static int dispatch_far(ApplyPlan* P, bool mask, int grp, int64_t wg, const V2<T>* q, const V2<T>* m, int64_t ldstr,
+                        int nrhs, V2<T>* part, int64_t* nsplit, hipStream_t s) {
+  switch (grp) {
+    case 1: return launch_far<T, 1>(P, mask, wg, q, m, ldstr, nrhs, part, nsplit, s);
+    case 2: return launch_far<T, 2>(P, mask, wg, q, m, ldstr, nrhs, part, nsplit, s);
+    case 4: return launch_far<T, 4>(P, mask, wg, q, m, ldstr, nrhs, part, nsplit, s);
+    case 8: return launch_far<T, 8>(P, mask, wg, q, m, ldstr, nrhs, part, nsplit, s);
+    case 16:
+      if constexpr (sizeof(T) == 4) return launch_far<T, 16>(P, mask, wg, q, m, ldstr, nrhs, part, nsplit, s);
+      return -6;
+    default: return -6;
+  }
+}
+
+// pot = far (+ CSR) for all right-hand sides
+template <typename T>
+static int far_and_csr(ApplyPlan* P, int prec, bool mask, const V2<T>* q, const V2<T>* m, const V2<T>* sg,
+                       V2<T>* pt, int nrhs, hipStream_t s, hipEvent_t ev_far) {
+  const int64_t N = P->npts, NO = P->nptso;
+  V2<T>* part = (V2<T>*)(sizeof(T) == 8 ? (void*)P->part64 : (void*)P->part32);
+  const V2<T>* wn = (const V2<T>*)(sizeof(T) == 8 ? (void*)P->wnear64 : (void*)P->wnear32);
+  int64_t nsplit = 0;
+  if (N == 0) return 0;
+  if (NO > 0) {
+    int64_t wg;
+    const int grp = far_group<T>(P, nrhs, prec, &wg);
+    const int rc = dispatch_far<T>(P, mask, grp, wg, q, m, NO, nrhs, part, &nsplit, s);
+    if (rc) return rc;
+  }
+  if (ev_far) ACHK(hipEventRecord(ev_far, s));
+  if (P->uniform_p > 0) {
+    k_reduce_csr<T, T, 8><<<nblk(N, 4), 256, 0, s>>>(N, nrhs, (int)nsplit, part, (int64_t)nrhs * N, P->row_ptr,
+                                                      P->col_ind, P->iquad, P->ixyzs, P->uniform_p, wn, sg, N, pt, N);
+  } else {
+    if (nsplit > 0)
+      k_farsum_reduce<T><<<dim3(nblk(N, 256), nrhs), 256, 0, s>>>(N, nrhs, (int)nsplit, part, (int64_t)nrhs * N, pt,
+                                                                  N);
+    else
+      k_fill<T><<<nblk(2 * N * nrhs, 256), 256, 0, s>>>(2 * N * nrhs, (T*)pt, (T)0);
+    k_csr_gen<T, T><<<nblk(N, 256), 256, 0, s>>>(N, P->row_ptr, P->col_ind, P->iquad, P->ixyzs, wn, sg, N, nrhs,
+                                                 pt, N);
+  }
+  ACHK(hipGetLastError());
+  return 0;
 }
 
 // sigma/pot: complex (interleaved) device arrays of length npts * nrhs, leading dim npts
@@ -576,6 +700,7 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
   if (tm)
     for (auto& e : ev) ACHK(hipEventCreate(&e));
   if (tm) ACHK(hipEventRecord(ev[0], s));
+  int rc = 0;
   if (prec == 64) {
     const V2<double>* sg = (const V2<double>*)sigma;
     V2<double>* pt = (V2<double>*)pot;
@@ -586,20 +711,9 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
     const bool mask = mode == APPLY_MASKED;
     auto* q = (const V2<double>*)P->q64;
     auto* m = (const V2<double>*)P->m64;
-    switch (nrhs) {
-      case 1: launch_far<double, 1>(P, mask, q, m, NO, pt, N, s); break;
-      default:
-        for (int r = 0; r < nrhs; r++) launch_far<double, 1>(P, mask, q + r * NO, m + r * NO, NO, pt + r * N, N, s);
-    }
-    if (tm) ACHK(hipEventRecord(ev[2], s));
-    if (P->uniform_p > 0)
-      k_csr<double, double><<<nblk(N, 4), 256, 0, s>>>(N, P->row_ptr, P->col_ind, P->iquad, P->ixyzs, P->uniform_p,
-                                                       (const V2<double>*)P->wnear64, sg, N, nrhs, pt, N);
-    else
-      k_csr_gen<double, double><<<nblk(N, 256), 256, 0, s>>>(N, P->row_ptr, P->col_ind, P->iquad, P->ixyzs,
-                                                             (const V2<double>*)P->wnear64, sg, N, nrhs, pt, N);
+    rc = far_and_csr<double>(P, 64, mask, q, m, sg, pt, nrhs, s, tm ? ev[2] : nullptr);
     if (tm) ACHK(hipEventRecord(ev[3], s));
-    if (!mask) {
+    if (!rc && !mask) {
       const double kr = P->helm ? P->zk[0] : 0.0, ki = P->helm ? P->zk[1] : 0.0;
       k_near_sub<<<nblk(N, 4), 256, 0, s>>>(N, P->trg64[0], P->trg64[1], P->trg64[2], P->src64[0], P->src64[1],
                                             P->src64[2], P->src64[3], P->src64[4], P->src64[5], q, m, NO, nrhs,
@@ -613,29 +727,12 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
                                                       sg, nrhs, N, P->wover32, ar, ai, br, bi, (V2<float>*)P->q32,
                                                       (V2<float>*)P->m32, NO);
     if (tm) ACHK(hipEventRecord(ev[1], s));
-    auto* q = (const V2<float>*)P->q32;
-    auto* m = (const V2<float>*)P->m32;
-    switch (nrhs) {
-      case 1: launch_far<float, 1>(P, true, q, m, NO, pt, N, s); break;
-      case 2: launch_far<float, 2>(P, true, q, m, NO, pt, N, s); break;
-      case 4: launch_far<float, 4>(P, true, q, m, NO, pt, N, s); break;
-      case 8: launch_far<float, 8>(P, true, q, m, NO, pt, N, s); break;
-      default: {
-        int r = 0;
-        for (; r + 8 <= nrhs; r += 8) launch_far<float, 8>(P, true, q + r * NO, m + r * NO, NO, pt + r * N, N, s);
-        for (; r < nrhs; r++) launch_far<float, 1>(P, true, q + r * NO, m + r * NO, NO, pt + r * N, N, s);
-      }
-    }
-    if (tm) ACHK(hipEventRecord(ev[2], s));
-    if (P->uniform_p > 0)
-      k_csr<float, float><<<nblk(N, 4), 256, 0, s>>>(N, P->row_ptr, P->col_ind, P->iquad, P->ixyzs, P->uniform_p,
-                                                     (const V2<float>*)P->wnear32, sg, N, nrhs, pt, N);
-    else
-      k_csr_gen<float, float><<<nblk(N, 256), 256, 0, s>>>(N, P->row_ptr, P->col_ind, P->iquad, P->ixyzs,
-                                                           (const V2<float>*)P->wnear32, sg, N, nrhs, pt, N);
+    rc = far_and_csr<float>(P, 32, true, (const V2<float>*)P->q32, (const V2<float>*)P->m32, sg, pt, nrhs, s,
+                            tm ? ev[2] : nullptr);
     if (tm) ACHK(hipEventRecord(ev[3], s));
     if (tm) ACHK(hipEventRecord(ev[4], s));
   }
+  if (rc) return rc;
   ACHK(hipGetLastError());
   if (tm) {
     ACHK(hipEventSynchronize(ev[4]));
@@ -647,6 +744,68 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
     for (auto& e : ev) (void)hipEventDestroy(e);
   }
   return 0;
+}
+
+// Autotuner of the far-sum launch shape for (prec, nrhs): every right-hand-side group
+// size that divides nrhs (1..8 in FP64, 1..16 in FP32) and workgroup targets of 1024,
+// 2048, 4096 are timed (one warm-up, best of two) on unit densities; the fastest is kept
+// in the plan and used by every later apply with the same (prec, nrhs).
+int apply_plan_tune(ApplyPlan* P, int prec, int nrhs, hipStream_t s, double* report, int nreport) {
+  if (nrhs < 1 || nrhs > 64 || (prec != 32 && prec != 64)) return -6;
+  if (ensure_ws(P, nrhs) != 0) return -2;
+  if (P->npts == 0 || P->nptso == 0) return 0;
+  const int64_t NO = P->nptso;
+  if (prec == 64) {
+    k_fill<double><<<nblk(2 * NO * nrhs, 256), 256, 0, s>>>(2 * NO * nrhs, P->q64, 1.0);
+    k_fill<double><<<nblk(2 * NO * nrhs, 256), 256, 0, s>>>(2 * NO * nrhs, P->m64, 1.0);
+  } else {
+    k_fill<float><<<nblk(2 * NO * nrhs, 256), 256, 0, s>>>(2 * NO * nrhs, P->q32, 1.0f);
+    k_fill<float><<<nblk(2 * NO * nrhs, 256), 256, 0, s>>>(2 * NO * nrhs, P->m32, 1.0f);
+  }
+  hipEvent_t e0, e1;
+  ACHK(hipEventCreate(&e0));
+  ACHK(hipEventCreate(&e1));
+  const int groups[5] = {1, 2, 4, 8, 16};
+  const int64_t wgs[3] = {1024, 2048, 4096};
+  float best = 1e30f;
+  int bg = 1, nrep = 0;
+  int64_t bw = P->far_wg_target;
+  for (int gsz : groups) {
+    if (nrhs % gsz != 0 || (prec == 64 && gsz > 8)) continue;
+    for (int64_t wg : wgs) {
+      float tbest = 1e30f;
+      for (int rep = 0; rep < 3; rep++) {
+        int64_t nsplit;
+        ACHK(hipEventRecord(e0, s));
+        int rc = prec == 64 ? dispatch_far<double>(P, true, gsz, wg, (const V2<double>*)P->q64,
+                                                   (const V2<double>*)P->m64, NO, nrhs, (V2<double>*)P->part64,
+                                                   &nsplit, s)
+                            : dispatch_far<float>(P, true, gsz, wg, (const V2<float>*)P->q32,
+                                                  (const V2<float>*)P->m32, NO, nrhs, (V2<float>*)P->part32, &nsplit,
+                                                  s);
+        if (rc) return rc;
+        ACHK(hipEventRecord(e1, s));
+        ACHK(hipEventSynchronize(e1));
+        float ms;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0 && ms < tbest) tbest = ms;
+      }
+      if (report && nrep + 3 <= nreport) {
+        report[nrep++] = gsz;
+        report[nrep++] = (double)wg;
+        report[nrep++] = tbest;
+      }
+      if (tbest < best) {
+        best = tbest;
+        bg = gsz;
+        bw = wg;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  P->tuned[prec * 1000 + nrhs] = {bg, bw};
+  return nrep / 3;
 }
 
 // real (Laplace) densities: convert to complex, apply, take real part

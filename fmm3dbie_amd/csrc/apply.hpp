@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <map>
 #include <vector>
 
 enum { APPLY_ADDSUB = 0, APPLY_MASKED = 1 };
@@ -48,7 +49,9 @@ struct ApplyPlan {
   float *q32 = nullptr, *m32 = nullptr, *part32 = nullptr;
   // far-sum launch shape: target workgroups x source splits >= far_wg_target
   int64_t far_wg_target = 2048;
-  int far_split_cap = 32;
+  int far_split_cap = 8;
+  // autotuned far-sum shape per (prec * 1000 + nrhs): (RHS group size, workgroup target)
+  std::map<int, std::pair<int, int64_t>> tuned;
   std::vector<void*> owned;
   ~ApplyPlan();
 };
@@ -59,3 +62,6 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
                    ApplyTimes* tm);
 int apply_plan_run_real(ApplyPlan* P, int mode, const double* sigma, double* pot, hipStream_t s);
 void apply_plan_free_ws(ApplyPlan* P);
+// time the far-sum launch shapes for (prec, nrhs) and keep the fastest; report gets
+// (group, workgroup target, ms) triples; returns the number of shapes timed
+int apply_plan_tune(ApplyPlan* P, int prec, int nrhs, hipStream_t s, double* report, int nreport);

@@ -447,6 +447,13 @@ int fmm3dbie_apply_plan_info(void* plan, int64_t* out4) {
 
 void fmm3dbie_apply_plan_destroy(void* plan) { delete (ApplyPlan*)plan; }
 
+int fmm3dbie_apply_plan_tune(void* plan, int prec, int nrhs, void* stream, double* report, int nreport) {
+  if (!plan) return -12;
+  const int rc = apply_plan_tune((ApplyPlan*)plan, prec, nrhs, (hipStream_t)stream, report, nreport);
+  g_last_error = rc < 0 ? rc : 0;
+  return rc;
+}
+
 // GMRES on a device-resident plan.  stats (optional, 2 ints): applies, FP64 applies.
 int fmm3dbie_gmres_dev(void* plan, int prec, int mode, const double* zid, const void* rhs, void* soln,
                        const double* wts, int nrhs, int numit, double eps_gmres, int* niter, double* errs,

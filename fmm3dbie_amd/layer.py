@@ -328,6 +328,17 @@ class ApplyPlan:
             times[:] = tm
         return pot
 
+    def tune(self, prec=32, nrhs=1, stream=None):
+        """Autotune the far-sum launch for (prec, nrhs); returns [(group, wg_target, ms)]."""
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rep = np.zeros(3 * 32)
+        n = lib().fmm3dbie_apply_plan_tune(self.handle, int(prec), int(nrhs), C.c_void_p(s.cuda_stream), _p(rep),
+                                           len(rep))
+        if n < 0:
+            raise RuntimeError("apply plan tuning failed (status %d)" % n)
+        return [(int(rep[3 * k]), int(rep[3 * k + 1]), float(rep[3 * k + 2])) for k in range(n)]
+
     def close(self):
         h = getattr(self, "handle", None)
         if h:

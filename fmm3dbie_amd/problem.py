@@ -49,3 +49,29 @@ def point_source(xyz, x0, zk=None):
 def plane_wave_dirs(n, seed):
     g = np.random.default_rng(seed).standard_normal((n, 3))
     return g / np.linalg.norm(g, axis=1, keepdims=True)
+
+
+def plane_waves(xyz, dirs, zk):
+    """u_inc[r, i] = exp(i k d_r . x_i) for unit directions d_r (rows)."""
+    return np.exp(1j * zk * (np.asarray(dirs) @ np.asarray(xyz).T))
+
+
+def mie_sound_soft(targets, dirs, zk, a=1.0, nmax=None):
+    """Field scattered by a sound-soft sphere of radius a centred at the origin for the
+    plane waves exp(i k d . x): u_s(x) = -sum_n (2n+1) i^n j_n(ka)/h_n(ka) h_n(k|x|)
+    P_n(d . x/|x|), h_n = j_n + i y_n (partial-wave expansion of the plane wave; the
+    series converges for |x| >= a once n exceeds ka by a margin)."""
+    from scipy.special import eval_legendre, spherical_jn, spherical_yn
+    targets = np.asarray(targets, float)
+    dirs = np.asarray(dirs, float)
+    r = np.linalg.norm(targets, axis=1)
+    ct = (dirs @ targets.T) / r[None, :]
+    ka = zk * a
+    if nmax is None:
+        nmax = int(ka + 10 * max(1.0, ka) ** (1 / 3) + 20)
+    u = np.zeros(ct.shape, complex)
+    for n in range(nmax + 1):
+        hna = spherical_jn(n, ka) + 1j * spherical_yn(n, ka)
+        hnr = spherical_jn(n, zk * r) + 1j * spherical_yn(n, zk * r)
+        u -= (2 * n + 1) * (1j ** n) * (spherical_jn(n, ka) / hna) * hnr[None, :] * eval_legendre(n, ct)
+    return u
