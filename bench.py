@@ -175,10 +175,10 @@ def main():
         x64, nit, errs, rres64, nap = layer.gmres(plan, rhs, zid, wts, numit=200, eps_gmres=1e-6)
         torch.cuda.synchronize()
         ts64 = time.perf_counter() - t0
-        layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=1e-4, tol=1e-6)
+        layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=1e-5, tol=1e-6)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        xir, outer, inner, rresir, n32, n64 = layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=1e-4, tol=1e-6)
+        xir, outer, inner, rresir, n32, n64 = layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=1e-5, tol=1e-6)
         torch.cuda.synchronize()
         tsir = time.perf_counter() - t0
         # exterior field error at 10 points on radius 3 vs the exact point-source field
@@ -271,7 +271,7 @@ def learned_mixed_solve(dev):
                                P["srcover"], P["whtsover"])
         plan.tune(32, 1)
         t0 = time.perf_counter()
-        x, outer, inner, rr, n32, n64 = layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-4, tol=1e-6)
+        x, outer, inner, rr, n32, n64 = layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-5, tol=1e-6)
         torch.cuda.synchronize()
         ts = time.perf_counter() - t0
         plan.close()
@@ -331,9 +331,9 @@ def batched_solve(dev, nseq):
     x64 = torch.stack([x[0] for x in xs]).cpu().numpy()
     tune, ttune = _timed(lambda: plan.tune(32, 64))
     best = min(tune, key=lambda t: t[2])
-    layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-4, tol=1e-6)
+    layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-5, tol=1e-6)
     (xb, outer, inner, rr, n32, n64), tb = _timed(
-        lambda: layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-4, tol=1e-6))
+        lambda: layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-5, tol=1e-6))
     xb = xb.cpu().numpy()
     rel = max(float(np.sqrt((np.abs(xb[r] - x64[r]) ** 2 * P["wts"]).sum() / (np.abs(x64[r]) ** 2 * P["wts"]).sum()))
               for r in range(nseq))

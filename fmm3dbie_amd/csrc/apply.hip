@@ -84,7 +84,7 @@ __device__ __forceinline__ void pair_kernel(T dx, T dy, T dz, T nx, T ny, T nz, 
   if constexpr (sizeof(T) == 4) {
     __sincosf(kr * r, &s, &c);
   } else {
-    sincos_cw(kr * r, &s, &c);
+    sincos(kr * r, &s, &c);
   }
   T mag = rinv;
   if (ki != (T)0) {

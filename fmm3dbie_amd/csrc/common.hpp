@@ -89,53 +89,8 @@ __device__ __forceinline__ void koorn_eval(double u, double v, double* P) {
 
 
 // ---- FP64 elementary functions for the kernel evaluations ------------------------
-// sin/cos: Cody-Waite reduction x = n*pi/2 + r with pi/2 split into three doubles
-// (the first two with 33 significant bits, so n*C1 and n*C2 are exact for |n| < 2^20),
-// then Taylor polynomials in r on [-pi/4, pi/4]: sin to r^17, cos to r^18 (truncation
-// below 1e-19).  Arguments beyond 2^20 fall back to the library routine.
-constexpr double cx_invfact(int n) {
-  double f = 1.0;
-  for (int i = 2; i <= n; i++) f *= i;
-  return 1.0 / f;
-}
-__device__ __attribute__((noinline)) void sincos_large(double x, double* sn, double* cs);
-
-__device__ __forceinline__ void sincos_cw(double x, double* sn, double* cs) {
-  if (!(fabs(x) < 1048576.0)) {
-    sincos_large(x, sn, cs);
-    return;
-  }
-  const double n = rint(x * 0x1.45f306dc9c883p-1);  // x * 2/pi
-  double r = fma(-n, 0x1.921fb54400000p+0, x);
-  r = fma(-n, 0x1.0b4611a600000p-34, r);
-  r = fma(-n, 0x1.3198a2e037073p-69, r);
-  const double r2 = r * r;
-  double ps = cx_invfact(17);
-  ps = fma(ps, r2, -cx_invfact(15));
-  ps = fma(ps, r2, cx_invfact(13));
-  ps = fma(ps, r2, -cx_invfact(11));
-  ps = fma(ps, r2, cx_invfact(9));
-  ps = fma(ps, r2, -cx_invfact(7));
-  ps = fma(ps, r2, cx_invfact(5));
-  ps = fma(ps, r2, -cx_invfact(3));
-  const double sr = fma(ps * r2, r, r);
-  double pc = cx_invfact(18);
-  pc = fma(pc, r2, -cx_invfact(16));
-  pc = fma(pc, r2, cx_invfact(14));
-  pc = fma(pc, r2, -cx_invfact(12));
-  pc = fma(pc, r2, cx_invfact(10));
-  pc = fma(pc, r2, -cx_invfact(8));
-  pc = fma(pc, r2, cx_invfact(6));
-  pc = fma(pc, r2, -cx_invfact(4));
-  pc = fma(pc, r2, cx_invfact(2));
-  const double cr = fma(-pc, r2, 1.0);
-  const int q = ((int)n) & 3;
-  const double a = (q & 1) ? cr : sr;
-  const double b = (q & 1) ? sr : cr;
-  *sn = (q & 2) ? -a : a;
-  *cs = ((q + 1) & 2) ? -b : b;
-}
-
+// sin/cos come from the device math library (sincos, inlined): measured faster here than
+// a hand-written Cody-Waite + Taylor version whose out-of-range fallback was a call.
 // 1/sqrt(x) for x > 0: hardware estimate refined by two Newton steps.
 __device__ __forceinline__ double rsqrt_nr(double x) {
   double y = __builtin_amdgcn_rsq(x);
@@ -162,7 +117,7 @@ __device__ __forceinline__ void helm_comb(double dx, double dy, double dz, doubl
   const double r = r2 * rinv;
   const double d = dx * nx + dy * ny + dz * nz;
   double s, c;
-  sincos_cw(kp.zkr * r, &s, &c);
+  sincos(kp.zkr * r, &s, &c);
   double er = c, ei = s;
   if (kp.zki != 0.0) {
     const double dec = exp(-kp.zki * r);

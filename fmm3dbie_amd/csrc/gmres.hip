@@ -130,6 +130,18 @@ __global__ void k_add(int64_t n, C2<double>* __restrict__ x, const C2<T>* __rest
   }
 }
 
+// x[r] += c[r] for the systems with upd[r] != 0 (grid y = system)
+template <typename T>
+__global__ void k_add_masked(int64_t n, C2<double>* __restrict__ x, const C2<T>* __restrict__ c,
+                             const int* __restrict__ upd) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (j < n && upd[r]) {
+    x[r * n + j].x += (double)c[r * n + j].x;
+    x[r * n + j].y += (double)c[r * n + j].y;
+  }
+}
+
 inline dim3 g2(int64_t n, int r) { return dim3((unsigned)((n + 255) / 256), (unsigned)r); }
 
 #define GCHK(x)                                                                    \
@@ -144,7 +156,7 @@ inline dim3 g2(int64_t n, int r) { return dim3((unsigned)((n + 255) / 256), (uns
 template <typename T>
 int gmres_impl(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, void* soln_v, const double* wts,
                int nrhs, int numit, double eps_gmres, int* niter, double* errs, double* rres, hipStream_t s,
-               GmresStats* gs) {
+               GmresStats* gs, const double* eps_rhs = nullptr) {
   using cd = std::complex<double>;
   const int64_t N = P->npts;
   const int prec = sizeof(T) == 8 ? 64 : 32;
@@ -233,7 +245,7 @@ int gmres_impl(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, voi
       sv[r * (numit + 1) + it - 1] = c * sv[r * (numit + 1) + it - 1];
       const double err = std::abs(sv[r * (numit + 1) + it]) / rb[r];
       errs[r * numit + it - 1] = err;
-      if (err <= eps_gmres || it == numit) {
+      if (err <= (eps_rhs ? eps_rhs[r] : eps_gmres) || it == numit) {
         done[r] = 1;
         its[r] = it;
       }
@@ -317,19 +329,30 @@ int gmres_ir_device(ApplyPlan* P, int mode, const double* zid, const void* rhs_v
   k_dots<double><<<dim3(1, nrhs), 256, 0, s>>>(N, 1, rhs, N, N, rhs, N, wts, dh, 2);
   GCHK(hipMemcpyAsync(hb.data(), dh, sizeof(double) * 4 * nrhs, hipMemcpyDeviceToHost, s));
   GCHK(hipStreamSynchronize(s));
-  std::vector<int> nit(nrhs);
-  std::vector<double> errs((size_t)numit * nrhs);
+  std::vector<int> nit(nrhs), upd(nrhs, 1);
+  std::vector<double> errs((size_t)numit * nrhs), eps_r(nrhs, eps_inner), cur(nrhs, 1.0);
+  int* dupd = nullptr;
+  GCHK(hipMalloc((void**)&dupd, sizeof(int) * nrhs));
   int outer = 0, inner = 0;
   double worst = 1.0;
   for (outer = 1; outer <= max_outer; outer++) {
+    // inner tolerance per system: eps_inner for the first solve, afterwards just the
+    // reduction still needed (with a factor 0.3 of margin) -- the inner residual is
+    // relative to the current FP64 residual; systems already at tol are left unchanged
+    for (int r = 0; r < nrhs; r++) {
+      upd[r] = cur[r] > tol ? 1 : 0;
+      const double need = 0.3 * tol / cur[r];
+      eps_r[r] = outer == 1 ? eps_inner : (upd[r] ? std::min(0.5, std::max(eps_inner, need)) : 0.5);
+    }
+    GCHK(hipMemcpyAsync(dupd, upd.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
     k_cvt<double, float><<<(unsigned)((N * nrhs + 255) / 256), 256, 0, s>>>(N * nrhs, r64, r32);
     int rc = gmres_impl<float>(P, mode, zid, r32, c32, wts, nrhs, numit, eps_inner, nit.data(), errs.data(),
-                               nullptr, s, gs);
+                               nullptr, s, gs, eps_r.data());
     if (rc) return rc;
     int mx = 0;
     for (int r = 0; r < nrhs; r++) mx = nit[r] > mx ? nit[r] : mx;
     inner += mx;
-    k_add<float><<<(unsigned)((N * nrhs + 255) / 256), 256, 0, s>>>(N * nrhs, x, c32);
+    k_add_masked<float><<<g2(N, nrhs), 256, 0, s>>>(N, x, c32, dupd);
     rc = apply_plan_run(P, 64, mode, x, kx, nrhs, s, nullptr);
     if (rc) return rc;
     if (gs) gs->napply64++;
@@ -341,10 +364,12 @@ int gmres_ir_device(ApplyPlan* P, int mode, const double* zid, const void* rhs_v
     for (int r = 0; r < nrhs; r++) {
       const double e = std::sqrt(hr[2 * (r * 2)]) / std::sqrt(hb[2 * (r * 2)]);
       if (rres) rres[r] = e;
+      cur[r] = e;
       worst = e > worst ? e : worst;
     }
     if (worst <= tol) break;
   }
+  (void)hipFree(dupd);
   if (outer > max_outer) outer = max_outer;
   if (outer_out) *outer_out = outer;
   if (inner_total) *inner_total = inner;

@@ -33,8 +33,6 @@
 __constant__ KoornConst g_koorn;
 extern __shared__ double fmm_dyn_lds[];
 
-// library sin/cos for the arguments outside the reduced-range fast path
-__device__ __attribute__((noinline)) void sincos_large(double x, double* sn, double* cs) { sincos(x, sn, cs); }
 
 namespace {
 
@@ -354,8 +352,9 @@ __global__ void k_tree_cache(int64_t npatches, int kq, const double* __restrict_
   }
 }
 
-// moments over cached tree triangle `tri`: lane q evaluates the kernel at node q, lane c
-// sums column c of (f_q * P_j(q)) with the basis values read from the shared table
+// moments over cached tree triangle `tri`: lane q evaluates the kernel at node q and
+// stages (f_q, P_j(q)) in LDS, the basis values copied from the shared table; lane c
+// then sums column c
 template <int N, bool HELM>
 __device__ __forceinline__ double tri_moments_cached(double* buf, const double* __restrict__ cpatch, int nn,
                                                      const double* __restrict__ ptab, int tri, int kq,
@@ -368,6 +367,8 @@ __device__ __forceinline__ double tri_moments_cached(double* buf, const double* 
     const int q = base + lane;
     if (q < kq) {
       const int node = tri * kq + q;
+      double* row = buf + lane * (NP + 2);
+      for (int l = 0; l < NP; l++) row[2 + l] = ptab[(size_t)node * NP + l];
       double g[12];
       g[0] = cpatch[node];
       g[1] = cpatch[nn + node];
@@ -378,28 +379,19 @@ __device__ __forceinline__ double tri_moments_cached(double* buf, const double* 
       const double w = cpatch[6 * nn + node];
       double vr, vi;
       kern_eval<HELM>(tg, g, kp, vr, vi);
-      buf[2 * lane] = vr * w;
-      buf[2 * lane + 1] = vi * w;
+      row[0] = vr * w;
+      row[1] = vi * w;
     }
     const int nrows = kq - base < WAVE ? kq - base : WAVE;
-    __syncthreads();
-    if (lane < NC) {
-      const int j = lane % NP, part = lane / NP;
-      const double* pcol = ptab + (size_t)(tri * kq + base) * NP + j;
-      double sacc = 0.0;
-#pragma unroll 4
-      for (int r = 0; r < nrows; r++) sacc = fma(buf[2 * r + part], pcol[(size_t)r * NP], sacc);
-      total += sacc;
-    }
-    __syncthreads();
+    total += column_sum<NP, HELM>(buf, nrows);
   }
   return total;
 }
 
 template <int N, bool HELM>
-__global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restrict__ list, int lpt,
+__global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restrict__ list,
                                                  const int2* __restrict__ fb_list, const int* __restrict__ fb_n,
-                                                 int* __restrict__ counter, int kq, const double* __restrict__ xg,
+                                                 int* __restrict__ qctr, int kq, const double* __restrict__ xg,
                                                  const int64_t* __restrict__ col_ind,
                                                  const int64_t* __restrict__ ixyzs,
                                                  const double* __restrict__ srccoefs, int64_t ndtarg,
@@ -417,18 +409,36 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
   const int lane = threadIdx.x;
   double* stack = stack_all + (size_t)blockIdx.x * STACK_CAP * ES;
   constexpr int NC = HELM ? 2 * NP : NP;
-  // work = the list (taken from its end when lpt: longest predicted first) followed by
-  // the pairs the cached-tree kernels sent back
-  const int npairs = nlist + (fb_n ? *fb_n : 0);
+  // Work queue: the list (sorted by patch) in 8 contiguous segments, one per XCD
+  // (workgroups are dispatched to the XCDs round-robin, so workgroup b starts on segment
+  // b % 8 and the patch geometry it reads stays in its XCD's L2), then the other
+  // segments once its own is drained, then the pairs the cached-tree kernels sent back.
+  const int nfb = fb_n ? *fb_n : 0;
+  const int home = blockIdx.x & 7;
+  int tried = 0;  // segments found empty (lane 0)
   for (;;) {
-    if (lane == 0) sh_idx = atomicAdd(counter, 1);
+    if (lane == 0) {
+      int got = -1;
+      while (tried < 9) {
+        const int sg = tried < 8 ? (home + tried) & 7 : 8;
+        const int a = sg < 8 ? (int)((int64_t)nlist * sg / 8) : 0;
+        const int b = sg < 8 ? (int)((int64_t)nlist * (sg + 1) / 8) : nfb;
+        const int k = atomicAdd(qctr + sg, 1);
+        if (a + k < b) {
+          got = sg < 8 ? a + k : nlist + k;
+          break;
+        }
+        tried++;
+      }
+      sh_idx = got;
+    }
     __syncthreads();
     // wave-uniform from here: the pair, its patch and the patch coefficients are read
     // through scalar loads into SGPRs
     const int idx = __builtin_amdgcn_readfirstlane(sh_idx);
     __syncthreads();
-    if (idx >= npairs) break;
-    const int2 jt = idx < nlist ? list[lpt ? nlist - 1 - idx : idx] : fb_list[idx - nlist];
+    if (idx < 0) break;
+    const int2 jt = idx < nlist ? list[idx] : fb_list[idx - nlist];
     const int64_t ip = col_ind[jt.x] - 1;
     const double* sc = srccoefs + 9 * (ixyzs[ip] - 1);
     const double* tp = targs + ndtarg * (int64_t)jt.y;
@@ -553,12 +563,19 @@ __global__ void k_features(int nadap, const int2* __restrict__ list, const int64
     for (int k = 0; k < 8; k++) feat_full[8 * (int64_t)jt.x + k] = f[k];
 }
 
-// bucket key = predicted class * npatches + patch: classes ascending, patches grouped
+// Bucket key of an adaptive pair (6 npatches keys): predicted classes 0 and 1 (the
+// cached-tree kernels) class-major then by patch; classes 2..5 (the adaptive kernel) by
+// patch, deeper predictions first within a patch.  Without a prediction every pair is in
+// the adaptive part, sorted by patch.
+__device__ __forceinline__ int64_t bucket_key(int c, int64_t patch, int64_t npat) {
+  return c < 2 ? c * npat + patch : 2 * npat + 4 * patch + (5 - c);
+}
+
 __global__ void k_bucket_count(int n, const int2* __restrict__ list, const int* __restrict__ pred,
                                const int64_t* __restrict__ col_ind, int64_t npat, int* __restrict__ cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  atomicAdd(cnt + pred[i] * npat + (col_ind[list[i].x] - 1), 1);
+  atomicAdd(cnt + bucket_key(pred ? pred[i] : 5, col_ind[list[i].x] - 1, npat), 1);
 }
 
 __global__ void k_bucket_scatter(int n, const int2* __restrict__ list, const int* __restrict__ pred,
@@ -567,7 +584,7 @@ __global__ void k_bucket_scatter(int n, const int2* __restrict__ list, const int
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int2 jt = list[i];
-  const int64_t key = pred[i] * npat + (col_ind[jt.x] - 1);
+  const int64_t key = bucket_key(pred ? pred[i] : 5, col_ind[jt.x] - 1, npat);
   out[off[key] + atomicAdd(fill + key, 1)] = jt;
   if (pred_full) pred_full[jt.x] = pred[i];
 }
@@ -587,39 +604,52 @@ __device__ __forceinline__ void tile_moments(dbl4 (&T)[HELM ? 2 : 1][((N + 1) * 
   constexpr int NP = (N + 1) * (N + 2) / 2;
   constexpr int CT = (NP + 15) / 16;
   const int lane = threadIdx.x, ia = lane & 15, ka = lane >> 4;
-#pragma unroll 1
-  for (int q0 = 0; q0 < kq; q0 += 4) {
-    const int q = q0 + ka;
-    double fr = 0.0, fi = 0.0, bv[CT];
+  // node data of the next step are loaded while the current step is evaluated
+  double g[12], w = 0.0, bv[CT];
+  auto load = [&](int q, double* gg, double& ww, double* bb) {
 #pragma unroll
-    for (int ct = 0; ct < CT; ct++) bv[ct] = 0.0;
+    for (int ct = 0; ct < CT; ct++) bb[ct] = 0.0;
+    ww = 0.0;
+    gg[0] = gg[1] = gg[2] = 0.0;
+    gg[9] = gg[10] = gg[11] = 0.0;
     if (q < kq) {
       const int node = tri * kq + q;
-      double g[12];
-      g[0] = cp[node];
-      g[1] = cp[nn + node];
-      g[2] = cp[2 * nn + node];
-      g[9] = cp[3 * nn + node];
-      g[10] = cp[4 * nn + node];
-      g[11] = cp[5 * nn + node];
-      const double w = cp[6 * nn + node];
-      double vr, vi;
-      kern_eval<HELM>(tg, g, kp, vr, vi);
-      if (row_ok) {
-        fr = vr * w;
-        fi = vi * w;
-      }
+      gg[0] = cp[node];
+      gg[1] = cp[nn + node];
+      gg[2] = cp[2 * nn + node];
+      gg[9] = cp[3 * nn + node];
+      gg[10] = cp[4 * nn + node];
+      gg[11] = cp[5 * nn + node];
+      ww = row_ok ? cp[6 * nn + node] : 0.0;
 #pragma unroll
       for (int ct = 0; ct < CT; ct++) {
         const int j = ia + 16 * ct;
-        if (j < NP) bv[ct] = ptab[(size_t)node * NP + j];
+        if (j < NP) bb[ct] = ptab[(size_t)node * NP + j];
       }
+    }
+  };
+  load(ka, g, w, bv);
+#pragma unroll 1
+  for (int q0 = 0; q0 < kq; q0 += 4) {
+    double gn[12], wn, bn[CT];
+    load(q0 + 4 + ka, gn, wn, bn);
+    double fr = 0.0, fi = 0.0;
+    if (q0 + ka < kq) {
+      double vr, vi;
+      kern_eval<HELM>(tg, g, kp, vr, vi);
+      fr = vr * w;
+      fi = vi * w;
     }
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
       T[0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, bv[ct], T[0][ct], 0, 0, 0);
       if (HELM) T[HELM ? 1 : 0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fi, bv[ct], T[HELM ? 1 : 0][ct], 0, 0, 0);
     }
+    g[0] = gn[0]; g[1] = gn[1]; g[2] = gn[2];
+    g[9] = gn[9]; g[10] = gn[10]; g[11] = gn[11];
+    w = wn;
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) bv[ct] = bn[ct];
   }
 }
 
@@ -663,15 +693,18 @@ __global__ void __launch_bounds__(64) k_shallow(int npairs, const int2* __restri
                                                 const double* __restrict__ umat, KPar kp,
                                                 const double* __restrict__ cache, const double* __restrict__ ptab,
                                                 int* __restrict__ nref_out, int2* __restrict__ fb_list,
-                                                int* __restrict__ counts, double* __restrict__ wnear) {
+                                                int* __restrict__ counts, double* __restrict__ wnear,
+                                                int chunk) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
   constexpr int CT = (NP + 15) / 16;
   constexpr int NPART = HELM ? 2 : 1;
   constexpr int RW = NPART * CT * 16;  // staged result row: [part][padded j]
-  __shared__ double res[16 * RW];
   __shared__ int code[16];
   const int lane = threadIdx.x, ia = lane & 15, ka = lane >> 4;
-  const int base = blockIdx.x * 16;
+  // workgroup b runs on XCD b % 8: give each XCD a contiguous run of 16-pair blocks (the
+  // list is sorted by patch) so that a patch's cached geometry is read into one L2
+  const int base = ((blockIdx.x & 7) * chunk + (blockIdx.x >> 3)) * 16;
+  if (base >= npairs) return;
   const int nn = NCACHE_TRI * kq;
   const bool row_ok = base + ia < npairs;
   const int2 jta = list[row_ok ? base + ia : base];
@@ -680,55 +713,90 @@ __global__ void __launch_bounds__(64) k_shallow(int npairs, const int2* __restri
   const double tg[3] = {tp[0], tp[1], tp[2]};
   const double* cp = cache + (size_t)ipa * 7 * nn;
   const dbl4 z4 = {0.0, 0.0, 0.0, 0.0};
-  dbl4 sumM[NPART][CT], T1[NPART][CT];
-#pragma unroll
-  for (int a = 0; a < NPART; a++)
-#pragma unroll
-    for (int ct = 0; ct < CT; ct++) sumM[a][ct] = z4;
-  double cmax[4] = {0.0, 0.0, 0.0, 0.0};
-  dbl4 sumS[NPART][CT];
-  if (DEPTH == 1) {
+  // LDS accumulators (each lane only touches its own elements, so no barrier is needed
+  // until the end): sM = sum of the children, sS = sum of the grandchildren, cM / cS =
+  // the current child and the sum of its children
+  __shared__ double sM[16 * RW], sS[DEPTH == 2 ? 16 * RW : 1], cM[DEPTH == 2 ? 16 * RW : 1],
+      cS[DEPTH == 2 ? 16 * RW : 1];
+  auto el = [&](int r, int a, int ct) { return (ka + 4 * r) * RW + a * CT * 16 + ct * 16 + ia; };
+  double cmax[4] = {0.0, 0.0, 0.0, 0.0}, e0[4] = {0.0, 0.0, 0.0, 0.0};
+  // triangles in order: DEPTH 1: children 1..4, root; DEPTH 2: each child followed by
+  // its four children (5 + 4 (c - 1) + g), then the root.  One tile call site keeps the
+  // kernel evaluation inlined once.
+  constexpr int NTASK = DEPTH == 1 ? 5 : 21;
 #pragma unroll 1
-    for (int c = 1; c <= 4; c++) tile_moments<N, HELM>(sumM, c, kq, nn, cp, ptab, tg, row_ok, kp);
-  } else {
-    dbl4 T2[NPART][CT];
+  for (int t = 0; t < NTASK; t++) {
+    int tri, c = 0, g = -1;
+    if (t == NTASK - 1) {
+      tri = 0;
+    } else if (DEPTH == 1) {
+      tri = 1 + t;
+      c = 1 + t;
+    } else {
+      c = 1 + t / 5;
+      g = t % 5 - 1;
+      tri = g < 0 ? c : 5 + 4 * (c - 1) + g;
+    }
+    dbl4 T[NPART][CT];
 #pragma unroll
     for (int a = 0; a < NPART; a++)
 #pragma unroll
-      for (int ct = 0; ct < CT; ct++) sumS[a][ct] = z4;
-#pragma unroll 1
-    for (int c = 1; c <= 4; c++) {
+      for (int ct = 0; ct < CT; ct++) T[a][ct] = z4;
+    tile_moments<N, HELM>(T, tri, kq, nn, cp, ptab, tg, row_ok, kp);
+    if (tri == 0) {
+      // root test: the children's sum against the root
+      dbl4 U[NPART][CT];
 #pragma unroll
-      for (int a = 0; a < NPART; a++)
+      for (int r = 0; r < 4; r++)
 #pragma unroll
-        for (int ct = 0; ct < CT; ct++) {
-          T1[a][ct] = z4;
-          T2[a][ct] = z4;
-        }
-      tile_moments<N, HELM>(T1, c, kq, nn, cp, ptab, tg, row_ok, kp);
-#pragma unroll 1
-      for (int g = 0; g < 4; g++) tile_moments<N, HELM>(T2, 5 + 4 * (c - 1) + g, kq, nn, cp, ptab, tg, row_ok, kp);
-      double e[4];
-      tile_err<NPART, CT>(T2, T1, e);
+        for (int a = 0; a < NPART; a++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) cmax[r] = e[r] > cmax[r] || e[r] != e[r] ? e[r] : cmax[r];
+          for (int ct = 0; ct < CT; ct++) U[a][ct][r] = sM[el(r, a, ct)];
+      tile_err<NPART, CT>(U, T, e0);
+    } else if (g < 0) {
+      // a child of the root
 #pragma unroll
-      for (int a = 0; a < NPART; a++)
+      for (int r = 0; r < 4; r++)
 #pragma unroll
-        for (int ct = 0; ct < CT; ct++) {
-          sumM[a][ct] += T1[a][ct];
-          sumS[a][ct] += T2[a][ct];
-        }
+        for (int a = 0; a < NPART; a++)
+#pragma unroll
+          for (int ct = 0; ct < CT; ct++) {
+            const int e = el(r, a, ct);
+            sM[e] = (c == 1 ? 0.0 : sM[e]) + T[a][ct][r];
+            if (DEPTH == 2) cM[e] = T[a][ct][r];
+          }
+    } else {
+      // a grandchild: accumulate; after the fourth, test the child
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int a = 0; a < NPART; a++)
+#pragma unroll
+          for (int ct = 0; ct < CT; ct++) {
+            const int e = el(r, a, ct);
+            cS[e] = (g == 0 ? 0.0 : cS[e]) + T[a][ct][r];
+          }
+      if (g == 3) {
+        dbl4 U[NPART][CT], V[NPART][CT];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+          for (int a = 0; a < NPART; a++)
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+              const int e = el(r, a, ct);
+              U[a][ct][r] = cS[e];
+              V[a][ct][r] = cM[e];
+              sS[e] = (c == 1 ? 0.0 : sS[e]) + cS[e];
+            }
+        double ec[4];
+        tile_err<NPART, CT>(U, V, ec);
+#pragma unroll
+        for (int r = 0; r < 4; r++) cmax[r] = ec[r] > cmax[r] || ec[r] != ec[r] ? ec[r] : cmax[r];
+      }
     }
   }
-#pragma unroll
-  for (int a = 0; a < NPART; a++)
-#pragma unroll
-    for (int ct = 0; ct < CT; ct++) T1[a][ct] = z4;
-  tile_moments<N, HELM>(T1, 0, kq, nn, cp, ptab, tg, row_ok, kp);
-  double e0[4];
-  tile_err<NPART, CT>(sumM, T1, e0);
-  // decide per accumulator row and stage the accepted moments
+  // decide per accumulator row; the accepted moments end up in sM
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int i = ka + 4 * r;
@@ -739,11 +807,12 @@ __global__ void __launch_bounds__(64) k_shallow(int npairs, const int2* __restri
       cd = !(e0[r] > eps) ? 1 : (DEPTH == 2 && !(cmax[r] > eps) ? 2 : 0);
     }
     if (ia == 0) code[i] = cd;
+    if (DEPTH == 2 && cd == 2) {
 #pragma unroll
-    for (int a = 0; a < NPART; a++)
+      for (int a = 0; a < NPART; a++)
 #pragma unroll
-      for (int ct = 0; ct < CT; ct++)
-        res[i * RW + a * CT * 16 + ct * 16 + ia] = (DEPTH == 2 && cd == 2) ? sumS[a][ct][r] : sumM[a][ct][r];
+        for (int ct = 0; ct < CT; ct++) sM[el(r, a, ct)] = sS[el(r, a, ct)];
+    }
   }
   __syncthreads();
   // weights = umat^T * moments for the accepted pairs
@@ -752,7 +821,7 @@ __global__ void __launch_bounds__(64) k_shallow(int npairs, const int2* __restri
     const int i = o / NOUT, rem = o - i * NOUT;
     if (code[i] <= 0) continue;
     const int j = rem / NPART, a = rem - j * NPART;
-    const double* m = res + i * RW + a * CT * 16;
+    const double* m = sM + i * RW + a * CT * 16;
     double sacc = 0.0;
     for (int l = 0; l < NP; l++) sacc = fma(umat[l + j * NP], m[l], sacc);
     const int64_t q0 = iquad[list[base + i].x] - 1;
@@ -1008,25 +1077,29 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   double* d_stack = dalloc<double>((size_t)nwaves * STACK_CAP * ES);
   if (!d_stack) return -2;
   int* d_over = d_counts + 2;
-  int* d_ctr = d_counts + 3;
-  // learned path buffers
+  // Adaptive pairs are bucketed by patch (and, with a selector, by predicted depth) so
+  // that consecutive work items share a patch's cached geometry.
   const bool learned = A.mlp_params != nullptr && N <= 6 && nadap > 0;
   const bool want_feat = (learned || A.feat_out) && nadap > 0;
   float* d_X = want_feat ? dalloc<float>((size_t)nadap * 8) : nullptr;
-  int *d_pred = nullptr, *d_boff = nullptr, *d_bcnt = nullptr, *d_pfull = nullptr;
+  int *d_pred = nullptr, *d_boff = nullptr, *d_bcnt = nullptr, *d_pfull = nullptr, *d_qctr = nullptr;
   int2 *d_sorted = nullptr, *d_fb = nullptr;
   if (want_feat && !d_X) return -2;
-  if (learned) {
-    d_pred = dalloc<int>(nadap);
+  if (nadap > 0) {
     d_bcnt = dalloc<int>(MLP_OUT * npat);
     d_boff = dalloc<int>(MLP_OUT * npat + 1);
     d_sorted = dalloc<int2>(nadap);
+    d_qctr = dalloc<int>(9);
+    if (!d_bcnt || !d_boff || !d_sorted || !d_qctr) return -2;
+  }
+  if (learned) {
+    d_pred = dalloc<int>(nadap);
     d_fb = dalloc<int2>(nadap);
     if (A.pred_host) d_pfull = dalloc<int>(A.nnz);
-    if (!d_pred || !d_bcnt || !d_boff || !d_sorted || !d_fb || (A.pred_host && !d_pfull)) return -2;
+    if (!d_pred || !d_fb || (A.pred_host && !d_pfull)) return -2;
   }
   HCHK(hipEventRecord(ev[5], s));
-  int nb[3] = {0, 0, 0};  // pairs predicted depth 1, depth 2, deeper
+  int nb[3] = {0, 0, nadap};  // pairs routed to depth 1, depth 2, the adaptive kernel
   if (want_feat) {
     const double kabs = HELM ? sqrt(A.zk[0] * A.zk[0] + A.zk[1] * A.zk[1]) : 0.0;
     k_features<N><<<(nadap + tb - 1) / tb, tb, 0, s>>>(nadap, d_adap, A.col_ind, A.ixyzs, A.srcvals, A.ndtarg,
@@ -1036,13 +1109,19 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   }
   if (learned) {
     if (mlp_predict_device(A.mlp_params, d_X, nadap, d_pred, nullptr, s) != 0) return -1;
-    HCHK(hipMemsetAsync(d_bcnt, 0, sizeof(int) * MLP_OUT * npat, s));
     if (d_pfull) HCHK(hipMemsetAsync(d_pfull, 0xff, sizeof(int) * A.nnz, s));
+  }
+  if (nadap > 0) {
+    HCHK(hipMemsetAsync(d_bcnt, 0, sizeof(int) * MLP_OUT * npat, s));
+    HCHK(hipMemsetAsync(d_qctr, 0, sizeof(int) * 9, s));
     k_bucket_count<<<(nadap + tb - 1) / tb, tb, 0, s>>>(nadap, d_adap, d_pred, A.col_ind, npat, d_bcnt);
     k_scan<<<1, 1024, 0, s>>>(MLP_OUT * npat, d_bcnt, d_boff);
     HCHK(hipMemsetAsync(d_bcnt, 0, sizeof(int) * MLP_OUT * npat, s));
     k_bucket_scatter<<<(nadap + tb - 1) / tb, tb, 0, s>>>(nadap, d_adap, d_pred, A.col_ind, npat, d_boff, d_bcnt,
                                                           d_sorted, d_pfull);
+    HCHK(hipGetLastError());
+  }
+  if (learned) {
     int b1 = 0, b2 = 0;
     HCHK(hipMemcpyAsync(&b1, d_boff + npat, sizeof(int), hipMemcpyDeviceToHost, s));
     HCHK(hipMemcpyAsync(&b2, d_boff + 2 * npat, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1056,26 +1135,24 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     k_tree_cache<N><<<npat, 256, 0, s>>>(npat, kq, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs, d_cache);
     HCHK(hipEventRecord(ev[8], s));
     if (learned) {
+      // grid = 8 x chunk workgroups, chunk = 16-pair blocks per XCD
+      const int c0 = ((nb[0] + 15) / 16 + 7) / 8, c1 = ((nb[1] + 15) / 16 + 7) / 8;
       if (nb[0] > 0)
-        k_shallow<N, HELM, 1><<<(nb[0] + 15) / 16, 64, 0, s>>>(nb[0], d_sorted, kq, A.col_ind, A.ndtarg, A.targs,
-                                                               A.iquad, d_epsp, d_umat, kp, d_cache, d_ptab, d_nref,
-                                                               d_fb, d_counts, A.wnear);
+        k_shallow<N, HELM, 1><<<8 * c0, 64, 0, s>>>(nb[0], d_sorted, kq, A.col_ind, A.ndtarg, A.targs, A.iquad,
+                                                    d_epsp, d_umat, kp, d_cache, d_ptab, d_nref, d_fb, d_counts,
+                                                    A.wnear, c0);
       if (nb[1] > 0)
-        k_shallow<N, HELM, 2><<<(nb[1] + 15) / 16, 64, 0, s>>>(nb[1], d_sorted + nb[0], kq, A.col_ind, A.ndtarg,
-                                                               A.targs, A.iquad, d_epsp, d_umat, kp, d_cache,
-                                                               d_ptab, d_nref, d_fb, d_counts, A.wnear);
+        k_shallow<N, HELM, 2><<<8 * c1, 64, 0, s>>>(nb[1], d_sorted + nb[0], kq, A.col_ind, A.ndtarg, A.targs,
+                                                    A.iquad, d_epsp, d_umat, kp, d_cache, d_ptab, d_nref, d_fb,
+                                                    d_counts, A.wnear, c1);
       HCHK(hipGetLastError());
     }
     HCHK(hipEventRecord(ev[7], s));
     const int g = nadap < nwaves ? nadap : nwaves;
-    if (learned)
-      k_adaptive<N, HELM><<<g, 64, 0, s>>>(nb[2], d_sorted + nb[0] + nb[1], 1, d_fb, d_counts + 4, d_ctr, kq, d_xg,
-                                           A.col_ind, A.ixyzs, A.srccoefs, A.ndtarg, A.targs, A.iquad, d_epsp,
-                                           d_umat, kp, d_stack, d_nref, d_over, A.wnear, d_cache, d_ptab);
-    else
-      k_adaptive<N, HELM><<<g, 64, 0, s>>>(nadap, d_adap, 0, nullptr, nullptr, d_ctr, kq, d_xg, A.col_ind,
-                                           A.ixyzs, A.srccoefs, A.ndtarg, A.targs, A.iquad, d_epsp, d_umat, kp,
-                                           d_stack, d_nref, d_over, A.wnear, d_cache, d_ptab);
+    k_adaptive<N, HELM><<<g, 64, 0, s>>>(nb[2], d_sorted + nb[0] + nb[1], d_fb, learned ? d_counts + 4 : nullptr,
+                                         d_qctr, kq, d_xg, A.col_ind, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
+                                         A.iquad, d_epsp, d_umat, kp, d_stack, d_nref, d_over, A.wnear, d_cache,
+                                         d_ptab);
     HCHK(hipGetLastError());
   } else {
     HCHK(hipEventRecord(ev[8], s));
@@ -1126,7 +1203,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   for (auto& e : ev) hipEventDestroy(e);
   void* frees[] = {d_umat, d_rvw, d_xg, d_fuv, d_Pq, d_Bq, d_cms, d_rads, d_epsp, d_cnt, d_off, d_fill,
                    d_cls, d_counts, d_adap, d_self, d_fixed, d_nref, d_stack, d_tuv, d_twq, d_ptab, d_cache,
-                   d_area, d_rvuv, d_X, d_pred, d_bcnt, d_boff, d_sorted, d_fb, d_pfull};
+                   d_area, d_rvuv, d_X, d_pred, d_bcnt, d_boff, d_sorted, d_fb, d_pfull, d_qctr};
   for (void* p : frees)
     if (p) hipFree(p);
   return 0;

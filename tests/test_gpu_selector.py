@@ -94,13 +94,15 @@ def test_trained_selector_parity_and_gain(cuda):
     nref0 = nq.stats["nref"].copy()
     params, info = S.train(X, y, n_train=int(0.7 * len(y)), epochs=30, seed=3, device=cuda)
     assert info["acc_heldout"] > 0.6, info
-    plain, learned = [], []
+    plain, learned, parts = [], [], []
     for _ in range(3):
         nq.run()
         plain.append(nq.stats["ms_tree"] + nq.stats["ms_adap"])
         nq.run(selector=params)
         st = nq.stats
         learned.append(st["ms_select"] + st["ms_tree"] + st["ms_shallow"] + st["ms_adap"])
+        parts.append((st["ms_select"], st["ms_tree"], st["ms_shallow"], st["ms_adap"]))
+    print("selector parts (select, tree, cached-tree kernels, adaptive) ms:", parts[int(np.argmin(learned))])
     w = nq.run(want_detail=True, selector=params).cpu().numpy()
     assert np.array_equal(nq.stats["nref"][idx], nref0[idx])
     assert np.abs(w - w0).max() < 1e-13 * np.abs(w0).max()
