@@ -41,6 +41,19 @@ def workload(cfg):
     raise SystemExit("config %d not wired into bench.py" % cfg)
 
 
+def measured_traffic(kernel, wname):
+    """HBM bytes per launch of `kernel` on workload `wname` from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), or None."""
+    import glob
+    for f in glob.glob(os.path.join(HERE, "profiles", "*.txt")):
+        for line in open(f):
+            if line.startswith("TRAFFIC_JSON "):
+                d = json.loads(line[len("TRAFFIC_JSON "):])
+                if d.get("kernel") == kernel and d.get("workload") == wname:
+                    return d
+    return None
+
+
 def pair_flops(stats, p, kq, nf, fk):
     """Algorithmic flops per launch of each quadrature kernel (adaptive: kq(1+4R)(F_K+4p)
     + 4p^2 per pair with R refinements; fixed: nf(F_K+4p) + 4p^2; self: n_s(F_K+22p+15)
@@ -153,6 +166,11 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None},
         "setup_host_s": t_setup_host,
     }
+    tr = measured_traffic("k_adaptive", wname) if name == "adaptive" else None
+    if tr is not None:
+        out["roofline"]["traffic"] = tr["traffic_bytes"]
+        out["roofline"]["traffic_source"] = "profiles/ PMC summary (FETCH_SIZE x %.2f + WRITE_SIZE x %.2f)" % (
+            tr["fetch_scale"], tr["write_scale"])
     # GMRES solves
     if not args.no_solve and helm:
         x0 = np.array([0.1, 0.2, -0.3])
@@ -161,6 +179,7 @@ def main():
         plan = layer.ApplyPlan(g, helm, pars, P["row_ptr"], P["col_ind"], P["iquad"], nq.wnear, P["novers"],
                                P["ixyzso"], P["srcover"], P["whtsover"])
         zid = 0.5
+        npts_, nptso, _, _ = plan.info()
         tm = np.zeros(4)
         s = rhs.clone()
         plan.apply(s, 64, 0)
@@ -175,10 +194,10 @@ def main():
         x64, nit, errs, rres64, nap = layer.gmres(plan, rhs, zid, wts, numit=200, eps_gmres=1e-6)
         torch.cuda.synchronize()
         ts64 = time.perf_counter() - t0
-        layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=1e-5, tol=1e-6)
+        layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=1e-4, tol=1e-6)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        xir, outer, inner, rresir, n32, n64 = layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=1e-5, tol=1e-6)
+        xir, outer, inner, rresir, n32, n64 = layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=1e-4, tol=1e-6)
         torch.cuda.synchronize()
         tsir = time.perf_counter() - t0
         # exterior field error at 10 points on radius 3 vs the exact point-source field
@@ -205,6 +224,9 @@ def main():
                         "ir_fp32_applies": n32, "ir_fp64_applies": n64, "ir_rres": float(rresir[0]),
                         "speedup_mixed_vs_fp64": ts64 / tsir, "field_err_fp64": e64, "field_err_mixed": eir,
                         "mixed_vs_fp64_solution_rel": rel_sol,
+                        "nptso": nptso,
+                        "far_fp64_tflops": npts_ * nptso * 44.0 / (t64[1] * 1e-3) / 1e12,
+                        "far_fp32_tflops": npts_ * nptso * 44.0 / (t32[1] * 1e-3) / 1e12,
                         "apply_fp64_ms": {"oversample": t64[0], "far": t64[1], "csr": t64[2], "subtract": t64[3]},
                         "apply_fp32_ms": {"oversample": t32[0], "far": t32[1], "csr": t32[2]}}
         plan.close()
@@ -271,7 +293,7 @@ def learned_mixed_solve(dev):
                                P["srcover"], P["whtsover"])
         plan.tune(32, 1)
         t0 = time.perf_counter()
-        x, outer, inner, rr, n32, n64 = layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-5, tol=1e-6)
+        x, outer, inner, rr, n32, n64 = layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-4, tol=1e-6)
         torch.cuda.synchronize()
         ts = time.perf_counter() - t0
         plan.close()
@@ -300,6 +322,8 @@ def learned_mixed_solve(dev):
             "learned_mixed": {"setup_plus_solve_s": ttl, "solve_s": tsl, "ir_outer": outer, "ir_inner_iters": inner,
                               "rres": rrl, "quad_ms": ql, "field_err": el, "solution_rel_vs_fp64": rel},
             "speedup_solve": ts64 / tsl, "speedup_setup_plus_solve": tt64 / ttl,
+            "adaptive_part_ms": {"plain": q64["ms_tree"] + q64["ms_adap"],
+                                 "learned": ql["ms_select"] + ql["ms_tree"] + ql["ms_shallow"] + ql["ms_adap"]},
             "selector": {"train_pairs": info["n_train"], "heldout_pairs": info["n_heldout"],
                          "acc_train": info["acc_train"], "acc_heldout": info["acc_heldout"],
                          "train_ms": info["ms_train"], "loss_first_last": [info["loss"][0], info["loss"][-1]]}}
@@ -331,9 +355,9 @@ def batched_solve(dev, nseq):
     x64 = torch.stack([x[0] for x in xs]).cpu().numpy()
     tune, ttune = _timed(lambda: plan.tune(32, 64))
     best = min(tune, key=lambda t: t[2])
-    layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-5, tol=1e-6)
+    layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-4, tol=1e-6)
     (xb, outer, inner, rr, n32, n64), tb = _timed(
-        lambda: layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-5, tol=1e-6))
+        lambda: layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-4, tol=1e-6))
     xb = xb.cpu().numpy()
     rel = max(float(np.sqrt((np.abs(xb[r] - x64[r]) ** 2 * P["wts"]).sum() / (np.abs(x64[r]) ** 2 * P["wts"]).sum()))
               for r in range(nseq))

@@ -610,8 +610,10 @@ __device__ __forceinline__ void tile_moments(dbl4 (&T)[HELM ? 2 : 1][((N + 1) * 
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) bb[ct] = 0.0;
     ww = 0.0;
-    gg[0] = gg[1] = gg[2] = 0.0;
-    gg[9] = gg[10] = gg[11] = 0.0;
+    for (int m = 0; m < 12; m++) gg[m] = 0.0;
+    gg[0] = tg[0] + 1.0;
+    gg[1] = tg[1];
+    gg[2] = tg[2];
     if (q < kq) {
       const int node = tri * kq + q;
       gg[0] = cp[node];
@@ -628,28 +630,46 @@ __device__ __forceinline__ void tile_moments(dbl4 (&T)[HELM ? 2 : 1][((N + 1) * 
       }
     }
   };
+  // two 4-node steps per iteration: two independent kernel evaluations per lane
+  double g2[12], w2, bv2[CT];
   load(ka, g, w, bv);
+  load(4 + ka, g2, w2, bv2);
 #pragma unroll 1
-  for (int q0 = 0; q0 < kq; q0 += 4) {
-    double gn[12], wn, bn[CT];
-    load(q0 + 4 + ka, gn, wn, bn);
-    double fr = 0.0, fi = 0.0;
-    if (q0 + ka < kq) {
-      double vr, vi;
+  for (int q0 = 0; q0 < kq; q0 += 8) {
+    double gn[12], wn, bn[CT], gn2[12], wn2, bn2[CT];
+    load(q0 + 8 + ka, gn, wn, bn);
+    load(q0 + 12 + ka, gn2, wn2, bn2);
+    double fr = 0.0, fi = 0.0, fr2 = 0.0, fi2 = 0.0;
+    {
+      double vr, vi, vr2, vi2;
       kern_eval<HELM>(tg, g, kp, vr, vi);
-      fr = vr * w;
-      fi = vi * w;
+      kern_eval<HELM>(tg, g2, kp, vr2, vi2);
+      // w = 0 marks a padding node or row; f must then be an exact zero
+      fr = w != 0.0 ? vr * w : 0.0;
+      fi = w != 0.0 ? vi * w : 0.0;
+      fr2 = w2 != 0.0 ? vr2 * w2 : 0.0;
+      fi2 = w2 != 0.0 ? vi2 * w2 : 0.0;
     }
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
       T[0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, bv[ct], T[0][ct], 0, 0, 0);
       if (HELM) T[HELM ? 1 : 0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fi, bv[ct], T[HELM ? 1 : 0][ct], 0, 0, 0);
+      T[0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr2, bv2[ct], T[0][ct], 0, 0, 0);
+      if (HELM)
+        T[HELM ? 1 : 0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fi2, bv2[ct], T[HELM ? 1 : 0][ct], 0, 0, 0);
     }
-    g[0] = gn[0]; g[1] = gn[1]; g[2] = gn[2];
-    g[9] = gn[9]; g[10] = gn[10]; g[11] = gn[11];
-    w = wn;
 #pragma unroll
-    for (int ct = 0; ct < CT; ct++) bv[ct] = bn[ct];
+    for (int m = 0; m < 12; m++) {
+      g[m] = gn[m];
+      g2[m] = gn2[m];
+    }
+    w = wn;
+    w2 = wn2;
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+      bv[ct] = bn[ct];
+      bv2[ct] = bn2[ct];
+    }
   }
 }
 

@@ -328,6 +328,12 @@ class ApplyPlan:
             times[:] = tm
         return pot
 
+    def info(self):
+        """(npts, nptso, nnz, uniform patch size or 0)."""
+        out = np.zeros(4, np.int64)
+        lib().fmm3dbie_apply_plan_info(self.handle, _p(out))
+        return tuple(int(x) for x in out)
+
     def tune(self, prec=32, nrhs=1, stream=None):
         """Autotune the far-sum launch for (prec, nrhs); returns [(group, wg_target, ms)]."""
         torch = _torch()
