@@ -254,13 +254,15 @@ __global__ void __launch_bounds__(128) k_fixed(int nf, const double* __restrict_
 
 // Moments of a set of staged nodes: row q of buf holds (f_re, f_im, P_0..P_{NP-1}) of
 // node q (written by the lane that evaluated it); lane c (< 2NP for complex) sums column c.
+// Single-group variant (lane c < NC sums column c over all rows): fewer registers, used
+// by the self kernel.
 template <int NP, bool HELM>
-__device__ __forceinline__ double column_sum(const double* buf, int nrows) {
+__device__ __forceinline__ double column_sum_serial(const double* buf, int nrows) {
   constexpr int W = NP + 2;
+  constexpr int NC = HELM ? 2 * NP : NP;
   const int lane = threadIdx.x;
   __syncthreads();
   double s = 0.0;
-  constexpr int NC = HELM ? 2 * NP : NP;
   if (lane < NC) {
     const int j = lane % NP, part = lane / NP;
 #pragma unroll 4
@@ -268,6 +270,29 @@ __device__ __forceinline__ double column_sum(const double* buf, int nrows) {
   }
   __syncthreads();
   return s;
+}
+
+// With NC = 2NP (complex) or NP columns, G = 64 / NC lane groups split the rows (group g
+// takes rows g, g + G, ...) and lane c adds the partial sums of lanes c + k NC.
+template <int NP, bool HELM>
+__device__ __forceinline__ double column_sum(const double* buf, int nrows) {
+  constexpr int W = NP + 2;
+  constexpr int NC = HELM ? 2 * NP : NP;
+  constexpr int G = 64 / NC;
+  const int lane = threadIdx.x;
+  __syncthreads();
+  double s = 0.0;
+  const int grp = lane / NC, c = lane - grp * NC;
+  if (grp < G) {
+    const int j = c % NP, part = c / NP;
+#pragma unroll 4
+    for (int q = grp; q < nrows; q += G) s = fma(buf[q * W + part], buf[q * W + 2 + j], s);
+  }
+  const double part_sum = s;  // shuffle the unmodified partial sums
+#pragma unroll
+  for (int k = 1; k < G; k++) s += __shfl(part_sum, (lane + k * NC) & 63, 64);
+  __syncthreads();
+  return lane < NC ? s : 0.0;
 }
 
 // Apply umat^T to per-lane moment components and store the weights of one pair.
@@ -318,12 +343,18 @@ __device__ __forceinline__ double tri_moments(double* buf, const double* sc, dou
   return total;
 }
 
-// Depth 0..2 of the refinement tree (1 + 4 + 16 triangles, numbered root 0, children
-// of the root 1..4, children of t in 1..4 at 5 + 4(t-1) + c) are the same for every
-// target of a patch, as in the reference's per-patch tree (triaintrouts.f:939-1033).
-// Their node geometry is computed once per patch into HBM (structure of arrays:
-// x, y, z, nx, ny, nz, |J|*da*w) and the basis values at those nodes once per order.
-constexpr int NCACHE_TRI = 21;
+// Depths 0..3 of the refinement tree (1 + 4 + 16 + 64 triangles: root 0, children of
+// the root 1..4, children of t in 1..4 at 5 + 4(t-1) + c, children of t in 5..20 at
+// 21 + 4(t-5) + c) are the same for every target of a patch, as in the reference's
+// per-patch tree (triaintrouts.f:939-1033).  Their node geometry is computed once per
+// patch into HBM (structure of arrays: x, y, z, nx, ny, nz, |J|*da*w) and the basis
+// values at those nodes once per order.
+constexpr int NCACHE_TRI = 85;
+constexpr int NCACHE_PARENT = 21;  // triangles whose children are cached (depths 0..2)
+
+__host__ __device__ __forceinline__ int cached_child(int t, int c) {
+  return t < 0 ? -1 : (t == 0 ? 1 + c : (t <= 4 ? 5 + 4 * (t - 1) + c : (t <= 20 ? 21 + 4 * (t - 5) + c : -1)));
+}
 
 template <int N>
 __global__ void k_tree_cache(int64_t npatches, int kq, const double* __restrict__ tuv, const double* __restrict__ twq,
@@ -471,7 +502,7 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
       const bool room = sp + 4 <= STACK_CAP;
 #pragma unroll 1
       for (int c = 0; c < 4; c++) {
-        const int ctid = tid < 0 ? -1 : (tid == 0 ? 1 + c : (tid <= 4 ? 5 + 4 * (tid - 1) + c : -1));
+        const int ctid = cached_child(tid, c);
         const double mcc = ctid >= 0
                                ? tri_moments_cached<N, HELM>(buf, cpatch, nn, ptab, ctid, kq, tg, kp)
                                : tri_moments<N, HELM>(buf, sc, cx[c], cy[c], cdx[c], cdy[c], da, kq, xg, tg, kp);
@@ -480,7 +511,7 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
           double* ne = stack + (size_t)(sp + c) * ES;
           if (lane == 0) {
             ne[0] = cx[c]; ne[1] = cy[c]; ne[2] = cdx[c]; ne[3] = cdy[c]; ne[4] = (double)(level + 1);
-            ne[5] = (double)(ctid >= 0 && ctid <= 4 ? ctid : -1);
+            ne[5] = (double)(ctid >= 0 && ctid < NCACHE_PARENT ? ctid : -1);
           }
           if (lane < NC) ne[6 + lane] = mcc;
         }
@@ -937,7 +968,7 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
             row[1] = vi * w;
           }
           const int nrows = nn - base < WAVE ? nn - base : WAVE;
-          acc += column_sum<NP, HELM>(buf, nrows);
+          acc += column_sum_serial<NP, HELM>(buf, nrows);
         }
       }
     }
@@ -1070,11 +1101,10 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
                         {t.x0 + hx, t.y0 + hy, -hx, -hy}};
       return k[c];
     };
-    for (int c = 0; c < 4; c++) tris[1 + c] = kids(tris[0], c);
-    for (int t = 1; t <= 4; t++)
-      for (int c = 0; c < 4; c++) tris[5 + 4 * (t - 1) + c] = kids(tris[t], c);
+    for (int t = 0; t < NCACHE_PARENT; t++)
+      for (int c = 0; c < 4; c++) tris[cached_child(t, c)] = kids(tris[t], c);
     for (int t = 0; t < NCACHE_TRI; t++) {
-      const double da = t == 0 ? 1.0 : (t <= 4 ? 0.25 : 0.0625);
+      const double da = t == 0 ? 1.0 : (t <= 4 ? 0.25 : (t <= 20 ? 0.0625 : 0.015625));
       for (int q = 0; q < kq; q++) {
         const int n = t * kq + q;
         tuv[2 * n] = tris[t].x0 + tris[t].dx * xq[2 * q];
