@@ -194,10 +194,10 @@ def main():
         x64, nit, errs, rres64, nap = layer.gmres(plan, rhs, zid, wts, numit=200, eps_gmres=1e-6)
         torch.cuda.synchronize()
         ts64 = time.perf_counter() - t0
-        layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=1e-4, tol=1e-6)
+        layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=5e-7, tol=1e-6)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        xir, outer, inner, rresir, n32, n64 = layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=1e-4, tol=1e-6)
+        xir, outer, inner, rresir, n32, n64 = layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=5e-7, tol=1e-6)
         torch.cuda.synchronize()
         tsir = time.perf_counter() - t0
         # exterior field error at 10 points on radius 3 vs the exact point-source field
@@ -233,6 +233,7 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, helm, pars, eps, P, args.cpu_seconds)
     if args.config == 1 and not args.no_solve:
+        out["laplace_solve"] = laplace_solve(dev)
         if not args.no_learned:
             out["learned_mixed_solve"] = learned_mixed_solve(dev)
         if not args.no_batched:
@@ -256,6 +257,39 @@ def _timed(fn):
     r = fn()
     torch.cuda.synchronize()
     return r, time.perf_counter() - t0
+
+
+def laplace_solve(dev):
+    """Laplace combined field (alpha = beta = 1) on the 4x4 cube sphere, order 3 (1920
+    points): near-quadrature rate and the exterior Dirichlet solve (GMRES tol 1e-7) for a
+    unit charge at (0.1, 0.2, -0.3), field checked at 10 points of radius 3."""
+    import torch
+    from fmm3dbie_amd import geometry as G, layer, problem
+    pars = np.array([1.0, 1.0])
+    g = G.get_sphere_npat(1.0, 4, (0, 0, 0), 3)
+    P = problem.setup(g, 1e-6, False, pars)
+    nq = layer.NearQuadDevice(g, False, pars, 1e-6, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], device=dev)
+    for _ in range(3):
+        nq.run()
+    _, tq = _timed(lambda: [nq.run() for _ in range(20)])
+    x0 = np.array([0.1, 0.2, -0.3])
+    rhs = torch.as_tensor(problem.point_source(g["srcvals"][:, :3], x0).astype(complex), device=dev)
+    wts = torch.as_tensor(P["wts"], device=dev)
+    plan = layer.ApplyPlan(g, False, pars, P["row_ptr"], P["col_ind"], P["iquad"], nq.wnear, P["novers"],
+                           P["ixyzso"], P["srcover"], P["whtsover"])
+    layer.gmres(plan, rhs, 0.5, wts, numit=100, eps_gmres=1e-7)
+    (x, nit, _, rres, _), ts = _timed(lambda: layer.gmres(plan, rhs, 0.5, wts, numit=100, eps_gmres=1e-7))
+    targ = 3.0 * problem.plane_wave_dirs(10, 0)
+    d = targ[:, None, :] - g["srcvals"][None, :, :3]
+    r = np.linalg.norm(d, axis=-1)
+    dn = (d * g["srcvals"][None, :, 9:12]).sum(-1)
+    ker = (dn / r ** 3 + 1.0 / r) / (4 * np.pi)
+    u = ker @ (x.cpu().numpy().real * P["wts"])
+    exact = problem.point_source(targ, x0)
+    plan.close()
+    return {"workload": "laplace_sphere_4x4_o3_pointcharge", "npts": len(g["srcvals"]), "nnz": P["nnz"],
+            "pairs_per_s": P["nnz"] / (tq / 20), "quad_ms": tq / 20 * 1e3, "gmres_s": ts, "iters": int(nit[0]),
+            "rres": float(rres[0]), "field_err": float(np.abs(u - exact).max() / np.abs(exact).max())}
 
 
 def learned_mixed_solve(dev):
@@ -293,7 +327,7 @@ def learned_mixed_solve(dev):
                                P["srcover"], P["whtsover"])
         plan.tune(32, 1)
         t0 = time.perf_counter()
-        x, outer, inner, rr, n32, n64 = layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-4, tol=1e-6)
+        x, outer, inner, rr, n32, n64 = layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=5e-7, tol=1e-6)
         torch.cuda.synchronize()
         ts = time.perf_counter() - t0
         plan.close()
@@ -355,9 +389,9 @@ def batched_solve(dev, nseq):
     x64 = torch.stack([x[0] for x in xs]).cpu().numpy()
     tune, ttune = _timed(lambda: plan.tune(32, 64))
     best = min(tune, key=lambda t: t[2])
-    layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-4, tol=1e-6)
+    layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=5e-7, tol=1e-6)
     (xb, outer, inner, rr, n32, n64), tb = _timed(
-        lambda: layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=1e-4, tol=1e-6))
+        lambda: layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=5e-7, tol=1e-6))
     xb = xb.cpu().numpy()
     rel = max(float(np.sqrt((np.abs(xb[r] - x64[r]) ** 2 * P["wts"]).sum() / (np.abs(x64[r]) ** 2 * P["wts"]).sum()))
               for r in range(nseq))

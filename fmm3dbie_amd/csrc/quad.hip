@@ -347,9 +347,10 @@ __device__ __forceinline__ double tri_moments(double* buf, const double* sc, dou
 // the root 1..4, children of t in 1..4 at 5 + 4(t-1) + c, children of t in 5..20 at
 // 21 + 4(t-5) + c) are the same for every target of a patch, as in the reference's
 // per-patch tree (triaintrouts.f:939-1033).  Their node geometry is computed once per
-// patch into HBM (structure of arrays: x, y, z, nx, ny, nz, |J|*da*w) and the basis
-// values at those nodes once per order.
+// patch into HBM (one 64-byte record per node: x, y, z, nx, ny, nz, |J|*da*w, pad, so a
+// node is read from one base address) and the basis values at those nodes once per order.
 constexpr int NCACHE_TRI = 85;
+constexpr int CACHE_STRIDE = 8;
 constexpr int NCACHE_PARENT = 21;  // triangles whose children are cached (depths 0..2)
 
 __host__ __device__ __forceinline__ int cached_child(int t, int c) {
@@ -364,7 +365,7 @@ __global__ void k_tree_cache(int64_t npatches, int kq, const double* __restrict_
   const int64_t ip = blockIdx.x;
   const int nn = NCACHE_TRI * kq;
   const double* sc = srccoefs + 9 * (ixyzs[ip] - 1);
-  double* out = cache + (size_t)ip * 7 * nn;
+  double* out = cache + (size_t)ip * CACHE_STRIDE * nn;
   for (int q = threadIdx.x; q < nn; q += blockDim.x) {
     double g[12];
     for (int m = 0; m < 9; m++) g[m] = 0.0;
@@ -373,13 +374,15 @@ __global__ void k_tree_cache(int64_t npatches, int kq, const double* __restrict_
       for (int m = 0; m < 9; m++) g[m] = fma(sc[9 * l + m], pl, g[m]);
     }
     const double jac = geom_finish(g);
-    out[0 * nn + q] = g[0];
-    out[1 * nn + q] = g[1];
-    out[2 * nn + q] = g[2];
-    out[3 * nn + q] = g[9];
-    out[4 * nn + q] = g[10];
-    out[5 * nn + q] = g[11];
-    out[6 * nn + q] = jac * twq[q];
+    double* o = out + (size_t)CACHE_STRIDE * q;
+    o[0] = g[0];
+    o[1] = g[1];
+    o[2] = g[2];
+    o[3] = g[9];
+    o[4] = g[10];
+    o[5] = g[11];
+    o[6] = jac * twq[q];
+    o[7] = 0.0;
   }
 }
 
@@ -401,13 +404,14 @@ __device__ __forceinline__ double tri_moments_cached(double* buf, const double* 
       double* row = buf + lane * (NP + 2);
       for (int l = 0; l < NP; l++) row[2 + l] = ptab[(size_t)node * NP + l];
       double g[12];
-      g[0] = cpatch[node];
-      g[1] = cpatch[nn + node];
-      g[2] = cpatch[2 * nn + node];
-      g[9] = cpatch[3 * nn + node];
-      g[10] = cpatch[4 * nn + node];
-      g[11] = cpatch[5 * nn + node];
-      const double w = cpatch[6 * nn + node];
+      const double* cn = cpatch + (size_t)CACHE_STRIDE * node;
+      g[0] = cn[0];
+      g[1] = cn[1];
+      g[2] = cn[2];
+      g[9] = cn[3];
+      g[10] = cn[4];
+      g[11] = cn[5];
+      const double w = cn[6];
       double vr, vi;
       kern_eval<HELM>(tg, g, kp, vr, vi);
       row[0] = vr * w;
@@ -476,7 +480,7 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
     const double tg[3] = {tp[0], tp[1], tp[2]};
     const double eps = epsp_patch[ip];
     const int nn = NCACHE_TRI * kq;
-    const double* cpatch = cache + (size_t)ip * 7 * nn;
+    const double* cpatch = cache + (size_t)ip * CACHE_STRIDE * nn;
     // root triangle (cached tree id 0)
     double mroot = tri_moments_cached<N, HELM>(buf, cpatch, nn, ptab, 0, kq, tg, kp);
     int sp = 0;
@@ -647,13 +651,14 @@ __device__ __forceinline__ void tile_moments(dbl4 (&T)[HELM ? 2 : 1][((N + 1) * 
     gg[2] = tg[2];
     if (q < kq) {
       const int node = tri * kq + q;
-      gg[0] = cp[node];
-      gg[1] = cp[nn + node];
-      gg[2] = cp[2 * nn + node];
-      gg[9] = cp[3 * nn + node];
-      gg[10] = cp[4 * nn + node];
-      gg[11] = cp[5 * nn + node];
-      ww = row_ok ? cp[6 * nn + node] : 0.0;
+      const double* cn = cp + (size_t)CACHE_STRIDE * node;
+      gg[0] = cn[0];
+      gg[1] = cn[1];
+      gg[2] = cn[2];
+      gg[9] = cn[3];
+      gg[10] = cn[4];
+      gg[11] = cn[5];
+      ww = row_ok ? cn[6] : 0.0;
 #pragma unroll
       for (int ct = 0; ct < CT; ct++) {
         const int j = ia + 16 * ct;
@@ -762,7 +767,7 @@ __global__ void __launch_bounds__(64) k_shallow(int npairs, const int2* __restri
   const int64_t ipa = col_ind[jta.x] - 1;
   const double* tp = targs + ndtarg * (int64_t)jta.y;
   const double tg[3] = {tp[0], tp[1], tp[2]};
-  const double* cp = cache + (size_t)ipa * 7 * nn;
+  const double* cp = cache + (size_t)ipa * CACHE_STRIDE * nn;
   const dbl4 z4 = {0.0, 0.0, 0.0, 0.0};
   // LDS accumulators (each lane only touches its own elements, so no barrier is needed
   // until the end): sM = sum of the children, sS = sum of the grandchildren, cM / cS =
@@ -1116,7 +1121,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     }
   }
   double *d_tuv = dalloc<double>(2 * ncn), *d_twq = dalloc<double>(ncn), *d_ptab = dalloc<double>((size_t)ncn * NP);
-  double* d_cache = nadap > 0 ? dalloc<double>((size_t)npat * 7 * ncn) : nullptr;
+  double* d_cache = nadap > 0 ? dalloc<double>((size_t)npat * CACHE_STRIDE * ncn) : nullptr;
   if (!d_tuv || !d_twq || !d_ptab || (nadap > 0 && !d_cache)) return -2;
   HCHK(hipMemcpyAsync(d_tuv, tuv.data(), sizeof(double) * 2 * ncn, hipMemcpyHostToDevice, s));
   HCHK(hipMemcpyAsync(d_twq, twq.data(), sizeof(double) * ncn, hipMemcpyHostToDevice, s));
