@@ -38,6 +38,10 @@ def workload(cfg):
         g = G.get_wavy_torus(48, 16, 6)
         zk = 3.0
         return g, True, np.array([zk, 1j * zk, 1.0]), "helmholtz_zk3_wavytorus_48x16_o6", zk
+    if cfg == 4:
+        g = G.get_deformed_sphere(32, 6)
+        zk = 10.0
+        return g, True, np.array([zk, 1j * zk, 1.0]), "helmholtz_zk10_deformedsphere_32x32_o6", zk
     raise SystemExit("config %d not wired into bench.py" % cfg)
 
 
@@ -172,7 +176,9 @@ def main():
         out["roofline"]["traffic_source"] = "profiles/ PMC summary (FETCH_SIZE x %.2f + WRITE_SIZE x %.2f)" % (
             tr["fetch_scale"], tr["write_scale"])
     # GMRES solves
-    if not args.no_solve and helm:
+    if args.config == 4:
+        out["apply"] = large_apply(g, pars, P, nq.wnear, dev)
+    elif not args.no_solve and helm:
         x0 = np.array([0.1, 0.2, -0.3])
         rhs = torch.as_tensor(problem.point_source(g["srcvals"][:, :3], x0, zk), device=dev)
         wts = torch.as_tensor(P["wts"], device=dev)
@@ -257,6 +263,34 @@ def _timed(fn):
     r = fn()
     torch.cuda.synchronize()
     return r, time.perf_counter() - t0
+
+
+def large_apply(g, pars, P, wnear, dev):
+    """Apply timings at the 344k-point configuration (no solve: one FP64 matvec costs
+    N x nptso pair interactions): FP64 and FP32 single right-hand side, FP32 with 16
+    right-hand sides after autotuning; the FP32 result against FP64."""
+    import torch
+    from fmm3dbie_amd import layer, problem
+    plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], wnear, P["novers"], P["ixyzso"],
+                           P["srcover"], P["whtsover"])
+    npts, nptso, _, _ = plan.info()
+    dirs = problem.plane_wave_dirs(16, 4)
+    zk = float(pars[0].real)
+    S = torch.as_tensor(-problem.plane_waves(g["srcvals"][:, :3], dirs, zk), device=dev)
+    t64, t32, t32b = np.zeros(4), np.zeros(4), np.zeros(4)
+    p64 = plan.apply(S[0], 64, 1, times=t64)
+    p32 = plan.apply(S[0].to(torch.complex64), 32, 1, times=t32)
+    rel = float((p32.to(torch.complex128) - p64).abs().max() / p64.abs().max())
+    tune = plan.tune(32, 16)
+    plan.apply(S.to(torch.complex64), 32, 1, times=t32b)
+    plan.close()
+    fl = npts * nptso * 44.0
+    return {"npts": npts, "nptso": nptso, "far_fp64_ms": t64[1], "far_fp32_ms": t32[1],
+            "far_fp32_16rhs_ms": t32b[1], "far_fp64_tflops": fl / (t64[1] * 1e-3) / 1e12,
+            "far_fp32_tflops": fl / (t32[1] * 1e-3) / 1e12,
+            "far_fp32_16rhs_tflops": npts * nptso * (20.0 + 24.0 * 16) / (t32b[1] * 1e-3) / 1e12,
+            "csr_fp64_ms": t64[2], "fp32_vs_fp64_rel": rel,
+            "tuned": min(tune, key=lambda t: t[2]) if tune else None}
 
 
 def laplace_solve(dev):
