@@ -233,6 +233,12 @@ def main():
                         "nptso": nptso,
                         "far_fp64_tflops": npts_ * nptso * 44.0 / (t64[1] * 1e-3) / 1e12,
                         "far_fp32_tflops": npts_ * nptso * 44.0 / (t32[1] * 1e-3) / 1e12,
+                        # near correction (fused with the far reduction): wnear + CSR index arrays +
+                        # row pointers + sigma and pot (the partial sums read are not counted)
+                        "csr_fp64_gbs": (16.0 * P["nquad"] + 16.0 * P["nnz"] + 8.0 * (N + 1) + 32.0 * N)
+                        / (t64[2] * 1e-3) / 1e9,
+                        "csr_fp32_gbs": (8.0 * P["nquad"] + 16.0 * P["nnz"] + 8.0 * (N + 1) + 16.0 * N)
+                        / (t32[2] * 1e-3) / 1e9,
                         "apply_fp64_ms": {"oversample": t64[0], "far": t64[1], "csr": t64[2], "subtract": t64[3]},
                         "apply_fp32_ms": {"oversample": t32[0], "far": t32[1], "csr": t32[2]}}
         plan.close()

@@ -236,39 +236,6 @@ __global__ void k_near_sub(int64_t ntarg, const double* __restrict__ tx, const d
   }
 }
 
-// CSR near correction, uniform patch size p: one wavefront per target row, lanes over
-// the row's contiguous wnear segment (coalesced), wave reduction.
-template <typename T, typename TW>
-__global__ void k_csr(int64_t ntarg, const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ind,
-                      const int64_t* __restrict__ iquad, const int64_t* __restrict__ ixyzs, int p,
-                      const V2<TW>* __restrict__ wnear, const V2<T>* __restrict__ sigma, int64_t ldsig, int nrhs,
-                      V2<T>* __restrict__ pot, int64_t ldpot) {
-  const int64_t i = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64;
-  const int lane = threadIdx.x % 64;
-  if (i >= ntarg) return;
-  const int64_t j0 = row_ptr[i] - 1, j1 = row_ptr[i + 1] - 1;
-  if (j0 == j1) return;
-  const int64_t q0 = iquad[j0] - 1;
-  const int64_t ne = (j1 - j0) * (int64_t)p;
-  for (int r = 0; r < nrhs; r++) {
-    double sr = 0.0, si = 0.0;
-    for (int64_t e = lane; e < ne; e += 64) {
-      const int64_t jj = j0 + e / p;
-      const int64_t src = ixyzs[col_ind[jj] - 1] - 1 + e % p;
-      const V2<TW> w = wnear[q0 + e];
-      const V2<T> s = sigma[r * ldsig + src];
-      sr += (double)w.x * (double)s.x - (double)w.y * (double)s.y;
-      si += (double)w.x * (double)s.y + (double)w.y * (double)s.x;
-    }
-    sr = wave_sum(sr);
-    si = wave_sum(si);
-    if (lane == 0) {
-      pot[r * ldpot + i].x += (T)sr;
-      pot[r * ldpot + i].y += (T)si;
-    }
-  }
-}
-
 // General CSR correction (any patch sizes): one thread per target row.
 template <typename T, typename TW>
 __global__ void k_csr_gen(int64_t ntarg, const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ind,
@@ -298,12 +265,14 @@ __global__ void k_csr_gen(int64_t ntarg, const int64_t* __restrict__ row_ptr, co
 // patch size p): one wavefront per target row, lanes over the row's contiguous wnear
 // segment; every wnear element is loaded once per group of RB right-hand sides.
 //   pot[r][i] = (sum over source splits of part[r][i]) / (4 pi) + sum_e wnear_e sigma_r(src_e)
-template <typename T, typename TW, int RB>
+// PC > 0: the patch size as a compile-time constant (the orders 1..10), else p_rt.
+template <typename T, typename TW, int RB, int PC>
 __global__ void k_reduce_csr(int64_t ntarg, int nrhs, int nsplit, const V2<T>* __restrict__ part, int64_t ldpart,
                              const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ind,
-                             const int64_t* __restrict__ iquad, const int64_t* __restrict__ ixyzs, int p,
+                             const int64_t* __restrict__ iquad, const int64_t* __restrict__ ixyzs, int p_rt,
                              const V2<TW>* __restrict__ wnear, const V2<T>* __restrict__ sigma, int64_t ldsig,
                              V2<T>* __restrict__ pot, int64_t ldpot) {
+  const int p = PC > 0 ? PC : p_rt;
   const int64_t i = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
   if (i >= ntarg) return;
@@ -314,9 +283,9 @@ __global__ void k_reduce_csr(int64_t ntarg, int nrhs, int nsplit, const V2<T>* _
     double ar[RB], ai[RB];
 #pragma unroll
     for (int r = 0; r < RB; r++) { ar[r] = 0.0; ai[r] = 0.0; }
-    for (int64_t e = lane; e < ne; e += 64) {
-      const int64_t jj = j0 + e / p;
-      const int64_t src = ixyzs[col_ind[jj] - 1] - 1 + e % p;
+    for (int e = lane; e < (int)ne; e += 64) {
+      const int k = e / p;  // patch entry of the row (compile-time divisor for the orders 1..10)
+      const int64_t src = ixyzs[col_ind[j0 + k] - 1] - 1 + (e - k * p);
       const V2<TW> w = wnear[q0 + e];
 #pragma unroll
       for (int r = 0; r < RB; r++) {
@@ -673,8 +642,23 @@ static int far_and_csr(ApplyPlan* P, int prec, bool mask, const V2<T>* q, const 
   }
   if (ev_far) ACHK(hipEventRecord(ev_far, s));
   if (P->uniform_p > 0) {
-    k_reduce_csr<T, T, 8><<<nblk(N, 4), 256, 0, s>>>(N, nrhs, (int)nsplit, part, (int64_t)nrhs * N, P->row_ptr,
-                                                      P->col_ind, P->iquad, P->ixyzs, P->uniform_p, wn, sg, N, pt, N);
+#define FMM_CSR(PC)                                                                                              \
+  k_reduce_csr<T, T, 8, PC><<<nblk(N, 4), 256, 0, s>>>(N, nrhs, (int)nsplit, part, (int64_t)nrhs * N, P->row_ptr, \
+                                                       P->col_ind, P->iquad, P->ixyzs, P->uniform_p, wn, sg, N, pt, N)
+    switch (P->uniform_p) {
+      case 3: FMM_CSR(3); break;
+      case 6: FMM_CSR(6); break;
+      case 10: FMM_CSR(10); break;
+      case 15: FMM_CSR(15); break;
+      case 21: FMM_CSR(21); break;
+      case 28: FMM_CSR(28); break;
+      case 36: FMM_CSR(36); break;
+      case 45: FMM_CSR(45); break;
+      case 55: FMM_CSR(55); break;
+      case 66: FMM_CSR(66); break;
+      default: FMM_CSR(0);
+    }
+#undef FMM_CSR
   } else {
     if (nsplit > 0)
       k_farsum_reduce<T><<<dim3(nblk(N, 256), nrhs), 256, 0, s>>>(N, nrhs, (int)nsplit, part, (int64_t)nrhs * N, pt,

@@ -928,6 +928,7 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
   const double vx[3] = {0.0, 1.0, 0.0}, vy[3] = {0.0, 0.0, 1.0};
   double acc = 0.0;
   const int nn = nr * nt;
+  const float inv_nr = 1.0f / (float)nr;
   for (int e = 0; e < 3; e++) {
     const double Ax = vx[e] - u0, Ay = vy[e] - v0;
     const double Bx = vx[(e + 1) % 3] - u0, By = vy[(e + 1) % 3] - v0;
@@ -955,9 +956,13 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
           const int k = base + lane;
           if (k < nn) {
             double* row = buf + lane * (NP + 2);
-            const int it = k / nr, ir = k % nr;
+            // k < 256: a float reciprocal gives the exact quotient
+            const int it = (int)(((float)k + 0.5f) * inv_nr), ir = k - it * nr;
             const double tau = t0 + (t1 - t0) * g_self_gl.x[it];
-            const double sh = sinh(tau), ch = cosh(tau);
+            // sinh and cosh from one exponential (only d * sinh enters, so the absolute
+            // error of the difference near tau = 0 is what matters)
+            const double et = exp(tau), eti = 1.0 / et;
+            const double sh = 0.5 * (et - eti), ch = 0.5 * (et + eti);
             const double wtau = g_self_gl.w[it] * (t1 - t0);
             const double px = fx + d * sh * ex, py = fy + d * sh * ey;
             const double rho = g_self_gl.x[ir];
@@ -973,7 +978,7 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
             row[1] = vi * w;
           }
           const int nrows = nn - base < WAVE ? nn - base : WAVE;
-          acc += column_sum_serial<NP, HELM>(buf, nrows);
+          acc += column_sum<NP, HELM>(buf, nrows);
         }
       }
     }
