@@ -268,8 +268,8 @@ __global__ void k_csr_gen(int64_t ntarg, const int64_t* __restrict__ row_ptr, co
 // PC > 0: the patch size as a compile-time constant (the orders 1..10), else p_rt.
 template <typename T, typename TW, int RB, int PC>
 __global__ void k_reduce_csr(int64_t ntarg, int nrhs, int nsplit, const V2<T>* __restrict__ part, int64_t ldpart,
-                             const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ind,
-                             const int64_t* __restrict__ iquad, const int64_t* __restrict__ ixyzs, int p_rt,
+                             const int64_t* __restrict__ row_ptr, const int* __restrict__ csr_src,
+                             const int64_t* __restrict__ iquad, int p_rt,
                              const V2<TW>* __restrict__ wnear, const V2<T>* __restrict__ sigma, int64_t ldsig,
                              V2<T>* __restrict__ pot, int64_t ldpot) {
   const int p = PC > 0 ? PC : p_rt;
@@ -285,7 +285,7 @@ __global__ void k_reduce_csr(int64_t ntarg, int nrhs, int nsplit, const V2<T>* _
     for (int r = 0; r < RB; r++) { ar[r] = 0.0; ai[r] = 0.0; }
     for (int e = lane; e < (int)ne; e += 64) {
       const int k = e / p;  // patch entry of the row (compile-time divisor for the orders 1..10)
-      const int64_t src = ixyzs[col_ind[j0 + k] - 1] - 1 + (e - k * p);
+      const int64_t src = csr_src[j0 + k] + (e - k * p);
       const V2<TW> w = wnear[q0 + e];
 #pragma unroll
       for (int r = 0; r < RB; r++) {
@@ -460,6 +460,9 @@ int apply_plan_build(ApplyPlan* P, const ApplyPlanInput& in, hipStream_t s) {
   P->np_of = own<int>(P, npat);
   P->opatch = own<int>(P, NO);
   P->near_sorted = own<int>(P, in.nnz);
+  P->csr_src = own<int>(P, in.nnz);
+  std::vector<int> csr_src(in.nnz);
+  for (int64_t j = 0; j < in.nnz; j++) csr_src[j] = (int)(in.ixyzs[in.col_ind[j] - 1] - 1);
   P->imat = own<double>(P, imat_all.size());
   P->wover = own<double>(P, NO);
   P->wover32 = own<float>(P, NO);
@@ -487,6 +490,7 @@ int apply_plan_build(ApplyPlan* P, const ApplyPlanInput& in, hipStream_t s) {
   ACHK(hipMemcpyAsync(P->np_of, np_of.data(), 4 * npat, hipMemcpyHostToDevice, s));
   ACHK(hipMemcpyAsync(P->opatch, opatch.data(), 4 * NO, hipMemcpyHostToDevice, s));
   ACHK(hipMemcpyAsync(P->near_sorted, nsorted.data(), 4 * in.nnz, hipMemcpyHostToDevice, s));
+  ACHK(hipMemcpyAsync(P->csr_src, csr_src.data(), 4 * in.nnz, hipMemcpyHostToDevice, s));
   ACHK(hipMemcpyAsync(P->imat, imat_all.data(), 8 * imat_all.size(), hipMemcpyHostToDevice, s));
   ACHK(hipMemcpyAsync(P->wover, in.whtsover, 8 * NO, hipMemcpyHostToDevice, s));
   ACHK(hipMemcpyAsync(d_srcover, in.srcover, 8 * 12 * NO, hipMemcpyHostToDevice, s));
@@ -642,9 +646,15 @@ static int far_and_csr(ApplyPlan* P, int prec, bool mask, const V2<T>* q, const 
   }
   if (ev_far) ACHK(hipEventRecord(ev_far, s));
   if (P->uniform_p > 0) {
-#define FMM_CSR(PC)                                                                                              \
-  k_reduce_csr<T, T, 8, PC><<<nblk(N, 4), 256, 0, s>>>(N, nrhs, (int)nsplit, part, (int64_t)nrhs * N, P->row_ptr, \
-                                                       P->col_ind, P->iquad, P->ixyzs, P->uniform_p, wn, sg, N, pt, N)
+#define FMM_CSR(PC)                                                                                             \
+  if (nrhs == 1)                                                                                               \
+    k_reduce_csr<T, T, 1, PC><<<nblk(N, 4), 256, 0, s>>>(N, nrhs, (int)nsplit, part, (int64_t)nrhs * N,         \
+                                                         P->row_ptr, P->csr_src, P->iquad, P->uniform_p, wn, sg, \
+                                                         N, pt, N);                                              \
+  else                                                                                                         \
+    k_reduce_csr<T, T, 8, PC><<<nblk(N, 4), 256, 0, s>>>(N, nrhs, (int)nsplit, part, (int64_t)nrhs * N,         \
+                                                         P->row_ptr, P->csr_src, P->iquad, P->uniform_p, wn, sg, \
+                                                         N, pt, N)
     switch (P->uniform_p) {
       case 3: FMM_CSR(3); break;
       case 6: FMM_CSR(6); break;

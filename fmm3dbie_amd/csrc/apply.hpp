@@ -34,6 +34,7 @@ struct ApplyPlan {
   int64_t *ixyzs = nullptr, *ixyzso = nullptr, *row_ptr = nullptr, *col_ind = nullptr, *iquad = nullptr,
           *ioff = nullptr;
   int *np_of = nullptr, *opatch = nullptr, *near_sorted = nullptr;
+  int* csr_src = nullptr;  // per CSR entry: first point (0-based) of its source patch
   double* imat = nullptr;
   double* wover = nullptr;
   float* wover32 = nullptr;
