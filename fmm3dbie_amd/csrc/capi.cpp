@@ -587,6 +587,9 @@ void lap_comb_dir_solver_guru_(const int64_t* npatches, const int64_t* norders, 
                    niter, errs, rres, soln);
 }
 
+// Free the device work buffers the near-quadrature calls keep between calls.
+void fmm3dbie_release_workspace() { fmm_nearquad_release(); }
+
 // ---------------------------------------------------------------- learned selector
 int fmm3dbie_mlp_nparam() { return MLP_NPARAM; }
 

@@ -23,6 +23,9 @@
 
 #include <cmath>
 #include <cstdio>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
@@ -994,6 +997,44 @@ T* dalloc(size_t n) {
   return (T*)p;
 }
 
+// Device workspace reused across calls: one grow-only buffer per slot and device.  Calls
+// are serialised (g_quad_mutex) and every call synchronises its stream before returning,
+// so a buffer is never resized while a kernel of an earlier call uses it.
+enum WsSlot {
+  W_CMS, W_RADS, W_EPSP, W_AREA, W_CNT, W_OFF, W_FILL, W_CLS, W_COUNTS, W_ADAP, W_SELF, W_FIXED, W_NREF,
+  W_CACHE, W_STACK, W_X, W_PRED, W_BCNT, W_BOFF, W_SORTED, W_FB, W_PFULL, W_QCTR, W_NSLOTS
+};
+struct WsBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_quad_mutex;
+std::map<int, std::vector<WsBuf>> g_ws;
+
+template <typename T>
+T* wsalloc(int slot, size_t n) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  auto& v = g_ws[dev];
+  if (v.size() < W_NSLOTS) v.resize(W_NSLOTS);
+  const size_t b = (n ? n : 1) * sizeof(T);
+  if (v[slot].bytes < b) {
+    if (v[slot].p) (void)hipFree(v[slot].p);
+    v[slot].p = nullptr;
+    v[slot].bytes = 0;
+    if (hipMalloc(&v[slot].p, b) != hipSuccess) return nullptr;
+    v[slot].bytes = b;
+  }
+  return (T*)v[slot].p;
+}
+
+// Patch-independent tables of one (order, rule) combination, built once per device.
+struct OrderTables {
+  int kq = 0, nf = 0;
+  double *umat, *rvw, *rvuv, *xg, *fuv, *Pq, *Bq, *tuv, *twq, *ptab;
+};
+std::map<std::tuple<int, int, int, int, int>, OrderTables> g_tables;
+
 #define HCHK(x)                                                                   \
   do {                                                                            \
     hipError_t e_ = (x);                                                          \
@@ -1021,14 +1062,18 @@ int set_self_gl(int n) {
   return 0;
 }
 
-template <int N, bool HELM>
-int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
+template <typename T>
+T* upload(const std::vector<T>& h, hipStream_t s) {
+  T* d = dalloc<T>(h.size());
+  if (d && hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s) != hipSuccess) return nullptr;
+  return d;
+}
+
+// umat, RV nodes/weights, the XG rule of the adaptive integrator, the fixed rule with its
+// basis (Pq) and weight-folded (Bq) tables, and the node tables of the cached tree
+template <int N>
+int build_tables(int nqorder, int nlev, int nqorder_f, OrderTables& T, hipStream_t s) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
-  const int64_t npat = A.npatches;
-  // tables
-  int nqorder, nlev, nqorder_f;
-  double eps_adap;
-  fmmhost::quadparams_adap(A.eps, A.norder_branch, &nqorder, &eps_adap, &nlev, &nqorder_f);
   std::vector<double> umat, rvuv(2 * NP), rvw(NP);
   fmmhost::rv_umat(N, umat);
   fmmhost::rv_nodes(N, rvuv.data(), rvw.data());
@@ -1046,27 +1091,84 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
       Bq[(size_t)q * NP + j] = fw[q] * sacc;
     }
   }
-  double *d_umat = dalloc<double>(NP * NP), *d_rvw = dalloc<double>(NP), *d_xg = dalloc<double>(3 * kq),
-         *d_fuv = dalloc<double>(2 * nf), *d_Pq = dalloc<double>((size_t)nf * NP),
-         *d_Bq = dalloc<double>((size_t)nf * NP);
-  double *d_cms = dalloc<double>(3 * npat), *d_rads = dalloc<double>(npat), *d_epsp = dalloc<double>(npat),
-         *d_area = dalloc<double>(npat), *d_rvuv = dalloc<double>(2 * NP);
-  int *d_cnt = dalloc<int>(npat), *d_off = dalloc<int>(npat + 1), *d_fill = dalloc<int>(npat);
-  int *d_cls = dalloc<int>(A.nnz), *d_counts = dalloc<int>(8);
-  int2 *d_adap = dalloc<int2>(A.nnz), *d_self = dalloc<int2>(A.nnz), *d_fixed = dalloc<int2>(A.nnz);
-  int* d_nref = A.want_nref ? dalloc<int>(A.nnz) : nullptr;
-  if (!d_umat || !d_rvw || !d_xg || !d_fuv || !d_Pq || !d_Bq || !d_cms || !d_rads || !d_epsp || !d_cnt ||
-      !d_off || !d_fill || !d_cls || !d_counts || !d_adap || !d_self || !d_fixed || !d_area || !d_rvuv) {
+  // node tables of the cached tree depths 0..3 (patch independent)
+  const int ncn = NCACHE_TRI * kq;
+  std::vector<double> tuv(2 * ncn), twq(ncn), ptab((size_t)ncn * NP);
+  {
+    struct Tri { double x0, y0, dx, dy; };
+    std::vector<Tri> tris(NCACHE_TRI);
+    tris[0] = {0.0, 0.0, 1.0, 1.0};
+    auto kids = [](Tri t, int c) -> Tri {
+      const double hx = t.dx / 2, hy = t.dy / 2;
+      const Tri k[4] = {{t.x0, t.y0, hx, hy}, {t.x0 + hx, t.y0, hx, hy}, {t.x0, t.y0 + hy, hx, hy},
+                        {t.x0 + hx, t.y0 + hy, -hx, -hy}};
+      return k[c];
+    };
+    for (int t = 0; t < NCACHE_PARENT; t++)
+      for (int c = 0; c < 4; c++) tris[cached_child(t, c)] = kids(tris[t], c);
+    for (int t = 0; t < NCACHE_TRI; t++) {
+      const double da = t == 0 ? 1.0 : (t <= 4 ? 0.25 : (t <= 20 ? 0.0625 : 0.015625));
+      for (int q = 0; q < kq; q++) {
+        const int n = t * kq + q;
+        tuv[2 * n] = tris[t].x0 + tris[t].dx * xq[2 * q];
+        tuv[2 * n + 1] = tris[t].y0 + tris[t].dy * xq[2 * q + 1];
+        twq[n] = da * xw[q];
+        fmmhost::koorn_pols(tuv[2 * n], tuv[2 * n + 1], N, pols.data());
+        for (int l = 0; l < NP; l++) ptab[(size_t)n * NP + l] = pols[l];
+      }
+    }
+  }
+  T.kq = kq;
+  T.nf = nf;
+  T.umat = upload(umat, s);
+  T.rvw = upload(rvw, s);
+  T.rvuv = upload(rvuv, s);
+  T.xg = upload(xg, s);
+  T.fuv = upload(fuv, s);
+  T.Pq = upload(Pq, s);
+  T.Bq = upload(Bq, s);
+  T.tuv = upload(tuv, s);
+  T.twq = upload(twq, s);
+  T.ptab = upload(ptab, s);
+  if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  if (!T.umat || !T.rvw || !T.rvuv || !T.xg || !T.fuv || !T.Pq || !T.Bq || !T.tuv || !T.twq || !T.ptab) return -2;
+  return 0;
+}
+
+template <int N, bool HELM>
+int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  const int64_t npat = A.npatches;
+  // tables (built and uploaded on the first call for this order and rule)
+  int nqorder, nlev, nqorder_f;
+  double eps_adap;
+  fmmhost::quadparams_adap(A.eps, A.norder_branch, &nqorder, &eps_adap, &nlev, &nqorder_f);
+  int dev = 0;
+  HCHK(hipGetDevice(&dev));
+  const auto key = std::make_tuple(dev, N, nqorder, nlev, nqorder_f);
+  auto tit = g_tables.find(key);
+  if (tit == g_tables.end()) {
+    OrderTables T;
+    if (build_tables<N>(nqorder, nlev, nqorder_f, T, s) != 0) return -2;
+    tit = g_tables.emplace(key, T).first;
+  }
+  const OrderTables& TB = tit->second;
+  const int kq = TB.kq, nf = TB.nf;
+  double *d_umat = TB.umat, *d_rvw = TB.rvw, *d_rvuv = TB.rvuv, *d_xg = TB.xg, *d_fuv = TB.fuv, *d_Pq = TB.Pq,
+         *d_Bq = TB.Bq;
+  double *d_cms = wsalloc<double>(W_CMS, 3 * npat), *d_rads = wsalloc<double>(W_RADS, npat),
+         *d_epsp = wsalloc<double>(W_EPSP, npat), *d_area = wsalloc<double>(W_AREA, npat);
+  int *d_cnt = wsalloc<int>(W_CNT, npat), *d_off = wsalloc<int>(W_OFF, npat + 1),
+      *d_fill = wsalloc<int>(W_FILL, npat);
+  int *d_cls = wsalloc<int>(W_CLS, A.nnz), *d_counts = wsalloc<int>(W_COUNTS, 8);
+  int2 *d_adap = wsalloc<int2>(W_ADAP, A.nnz), *d_self = wsalloc<int2>(W_SELF, A.nnz),
+       *d_fixed = wsalloc<int2>(W_FIXED, A.nnz);
+  int* d_nref = A.want_nref ? wsalloc<int>(W_NREF, A.nnz) : nullptr;
+  if (!d_cms || !d_rads || !d_epsp || !d_cnt || !d_off || !d_fill || !d_cls || !d_counts || !d_adap || !d_self ||
+      !d_fixed || !d_area || (A.want_nref && !d_nref)) {
     fprintf(stderr, "fmm3dbie_amd: device allocation failed in getnearquad\n");
     return -2;
   }
-  HCHK(hipMemcpyAsync(d_umat, umat.data(), sizeof(double) * NP * NP, hipMemcpyHostToDevice, s));
-  HCHK(hipMemcpyAsync(d_rvw, rvw.data(), sizeof(double) * NP, hipMemcpyHostToDevice, s));
-  HCHK(hipMemcpyAsync(d_rvuv, rvuv.data(), sizeof(double) * 2 * NP, hipMemcpyHostToDevice, s));
-  HCHK(hipMemcpyAsync(d_xg, xg.data(), sizeof(double) * 3 * kq, hipMemcpyHostToDevice, s));
-  HCHK(hipMemcpyAsync(d_fuv, fuv.data(), sizeof(double) * 2 * nf, hipMemcpyHostToDevice, s));
-  HCHK(hipMemcpyAsync(d_Pq, Pq.data(), sizeof(double) * nf * NP, hipMemcpyHostToDevice, s));
-  HCHK(hipMemcpyAsync(d_Bq, Bq.data(), sizeof(double) * nf * NP, hipMemcpyHostToDevice, s));
   HCHK(hipMemsetAsync(d_cnt, 0, sizeof(int) * npat, s));
   HCHK(hipMemsetAsync(d_fill, 0, sizeof(int) * npat, s));
   HCHK(hipMemsetAsync(d_counts, 0, sizeof(int) * 8, s));
@@ -1098,64 +1200,35 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
                                                 A.iquad, d_off, d_fixed, kp, A.wnear);
   HCHK(hipGetLastError());
   HCHK(hipEventRecord(ev[2], s));
-  // node tables of the cached tree depths 0..2 (patch independent)
   const int ncn = NCACHE_TRI * kq;
-  std::vector<double> tuv(2 * ncn), twq(ncn), ptab((size_t)ncn * NP);
-  {
-    struct Tri { double x0, y0, dx, dy; };
-    std::vector<Tri> tris(NCACHE_TRI);
-    tris[0] = {0.0, 0.0, 1.0, 1.0};
-    auto kids = [](Tri t, int c) -> Tri {
-      const double hx = t.dx / 2, hy = t.dy / 2;
-      const Tri k[4] = {{t.x0, t.y0, hx, hy}, {t.x0 + hx, t.y0, hx, hy}, {t.x0, t.y0 + hy, hx, hy},
-                        {t.x0 + hx, t.y0 + hy, -hx, -hy}};
-      return k[c];
-    };
-    for (int t = 0; t < NCACHE_PARENT; t++)
-      for (int c = 0; c < 4; c++) tris[cached_child(t, c)] = kids(tris[t], c);
-    for (int t = 0; t < NCACHE_TRI; t++) {
-      const double da = t == 0 ? 1.0 : (t <= 4 ? 0.25 : (t <= 20 ? 0.0625 : 0.015625));
-      for (int q = 0; q < kq; q++) {
-        const int n = t * kq + q;
-        tuv[2 * n] = tris[t].x0 + tris[t].dx * xq[2 * q];
-        tuv[2 * n + 1] = tris[t].y0 + tris[t].dy * xq[2 * q + 1];
-        twq[n] = da * xw[q];
-        fmmhost::koorn_pols(tuv[2 * n], tuv[2 * n + 1], N, pols.data());
-        for (int l = 0; l < NP; l++) ptab[(size_t)n * NP + l] = pols[l];
-      }
-    }
-  }
-  double *d_tuv = dalloc<double>(2 * ncn), *d_twq = dalloc<double>(ncn), *d_ptab = dalloc<double>((size_t)ncn * NP);
-  double* d_cache = nadap > 0 ? dalloc<double>((size_t)npat * CACHE_STRIDE * ncn) : nullptr;
-  if (!d_tuv || !d_twq || !d_ptab || (nadap > 0 && !d_cache)) return -2;
-  HCHK(hipMemcpyAsync(d_tuv, tuv.data(), sizeof(double) * 2 * ncn, hipMemcpyHostToDevice, s));
-  HCHK(hipMemcpyAsync(d_twq, twq.data(), sizeof(double) * ncn, hipMemcpyHostToDevice, s));
-  HCHK(hipMemcpyAsync(d_ptab, ptab.data(), sizeof(double) * ncn * NP, hipMemcpyHostToDevice, s));
+  double *d_tuv = TB.tuv, *d_twq = TB.twq, *d_ptab = TB.ptab;
+  double* d_cache = nadap > 0 ? wsalloc<double>(W_CACHE, (size_t)npat * CACHE_STRIDE * ncn) : nullptr;
+  if (nadap > 0 && !d_cache) return -2;
   // adaptive
   const int nwaves = A.adaptive_waves > 0 ? A.adaptive_waves : 4096;
   constexpr int ES = 6 + 2 * NP;
-  double* d_stack = dalloc<double>((size_t)nwaves * STACK_CAP * ES);
+  double* d_stack = wsalloc<double>(W_STACK, (size_t)nwaves * STACK_CAP * ES);
   if (!d_stack) return -2;
   int* d_over = d_counts + 2;
   // Adaptive pairs are bucketed by patch (and, with a selector, by predicted depth) so
   // that consecutive work items share a patch's cached geometry.
   const bool learned = A.mlp_params != nullptr && N <= 6 && nadap > 0;
   const bool want_feat = (learned || A.feat_out) && nadap > 0;
-  float* d_X = want_feat ? dalloc<float>((size_t)nadap * 8) : nullptr;
+  float* d_X = want_feat ? wsalloc<float>(W_X, (size_t)nadap * 8) : nullptr;
   int *d_pred = nullptr, *d_boff = nullptr, *d_bcnt = nullptr, *d_pfull = nullptr, *d_qctr = nullptr;
   int2 *d_sorted = nullptr, *d_fb = nullptr;
   if (want_feat && !d_X) return -2;
   if (nadap > 0) {
-    d_bcnt = dalloc<int>(MLP_OUT * npat);
-    d_boff = dalloc<int>(MLP_OUT * npat + 1);
-    d_sorted = dalloc<int2>(nadap);
-    d_qctr = dalloc<int>(9);
+    d_bcnt = wsalloc<int>(W_BCNT, MLP_OUT * npat);
+    d_boff = wsalloc<int>(W_BOFF, MLP_OUT * npat + 1);
+    d_sorted = wsalloc<int2>(W_SORTED, nadap);
+    d_qctr = wsalloc<int>(W_QCTR, 9);
     if (!d_bcnt || !d_boff || !d_sorted || !d_qctr) return -2;
   }
   if (learned) {
-    d_pred = dalloc<int>(nadap);
-    d_fb = dalloc<int2>(nadap);
-    if (A.pred_host) d_pfull = dalloc<int>(A.nnz);
+    d_pred = wsalloc<int>(W_PRED, nadap);
+    d_fb = wsalloc<int2>(W_FB, nadap);
+    if (A.pred_host) d_pfull = wsalloc<int>(W_PFULL, A.nnz);
     if (!d_pred || !d_fb || (A.pred_host && !d_pfull)) return -2;
   }
   HCHK(hipEventRecord(ev[5], s));
@@ -1261,11 +1334,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     if (A.class_host) hipMemcpy(A.class_host, d_cls, sizeof(int) * A.nnz, hipMemcpyDeviceToHost);
   }
   for (auto& e : ev) hipEventDestroy(e);
-  void* frees[] = {d_umat, d_rvw, d_xg, d_fuv, d_Pq, d_Bq, d_cms, d_rads, d_epsp, d_cnt, d_off, d_fill,
-                   d_cls, d_counts, d_adap, d_self, d_fixed, d_nref, d_stack, d_tuv, d_twq, d_ptab, d_cache,
-                   d_area, d_rvuv, d_X, d_pred, d_bcnt, d_boff, d_sorted, d_fb, d_pfull, d_qctr};
-  for (void* p : frees)
-    if (p) hipFree(p);
+  (void)d_umat;
   return 0;
 }
 
@@ -1291,6 +1360,7 @@ int dispatch(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
 }  // namespace
 
 int fmm_nearquad_device(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(g_quad_mutex);
   if (ensure_constants() != 0) return -1;
   if (set_self_gl(A.self_n) != 0) return -1;
   if (A.self_n < 1 || A.self_n > 16) {
@@ -1301,3 +1371,11 @@ int fmm_nearquad_device(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s)
 }
 
 int fmm_koorn_constants() { return ensure_constants(); }
+
+void fmm_nearquad_release() {
+  std::lock_guard<std::mutex> lock(g_quad_mutex);
+  for (auto& kv : g_ws)
+    for (auto& b : kv.second)
+      if (b.p) (void)hipFree(b.p);
+  g_ws.clear();
+}

@@ -48,5 +48,8 @@ struct NearQuadStats {
   double ms_tables, ms_select, ms_tree, ms_shallow;
 };
 
+// Calls are serialised; device work buffers and per-order tables are kept between calls
+// (grow-only) until fmm_nearquad_release().
 int fmm_nearquad_device(const NearQuadArgs& a, NearQuadStats* st, hipStream_t s);
+void fmm_nearquad_release();
 int fmm_koorn_constants();

@@ -171,6 +171,11 @@ int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatc
                           void *stream, double *stats, int *nref_host, int *class_host, const float *mlp_params,
                           float *feat_out, int *pred_host);
 
+/* Near-quadrature calls keep their device work buffers (grow-only, per device) and
+ * per-order tables between calls and are serialised by a lock; this frees the buffers.
+ * No reference counterpart. */
+void fmm3dbie_release_workspace(void);
+
 /* ---- learned refinement-depth selector (no reference counterpart) --------------------
  * MLP 8 -> 64 -> 64 -> 6 (ReLU, softmax), FP32, parameters packed as W1[64][8] b1[64]
  * W2[64][64] b2[64] W3[6][64] b3[6]; X is n x 8 row-major, y class ids 0..5 (class =
