@@ -178,6 +178,14 @@ def main():
     # GMRES solves
     if args.config == 4:
         out["apply"] = large_apply(g, pars, P, nq.wnear, dev)
+        from fmm3dbie_amd import near
+        rfac, _ = near.get_rfacs(near.norder_avg(g))
+        near.findnear_gpu(P["cms"], P["rads"] * rfac, g["srcvals"][:, :3])
+        t0 = time.perf_counter()
+        rpg, cig = near.findnear_gpu(P["cms"], P["rads"] * rfac, g["srcvals"][:, :3])
+        tg = time.perf_counter() - t0
+        out["findnear"] = {"gpu_s_incl_copies": tg, "same_as_setup": bool(np.array_equal(rpg, P["row_ptr"])
+                                                                            and np.array_equal(cig, P["col_ind"]))}
     elif not args.no_solve and helm:
         x0 = np.array([0.1, 0.2, -0.3])
         rhs = torch.as_tensor(problem.point_source(g["srcvals"][:, :3], x0, zk), device=dev)

@@ -27,6 +27,10 @@ SIGS = {
                                f64, vp, i32, f64, vp, vp, vp, vp, vp, vp, vp], i32),
     "fmm3dbie_mlp_nparam": ([], i32),
     "fmm3dbie_release_workspace": ([], None),
+    "findnearmem_": ([vp, vp, vp, vp, vp, vp, vp], None),
+    "findnear_": ([vp, vp, vp, vp, vp, vp, vp, vp], None),
+    "fmm3dbie_findnearmem_dev": ([i64, vp, vp, i64, i64, vp, vp, vp, vp], i32),
+    "fmm3dbie_findnear_dev": ([i64, vp, vp, i64, i64, vp, vp, vp, vp], i32),
     "fmm3dbie_mlp_grad": ([vp, vp, vp, i64, vp, vp, vp], i32),
     "fmm3dbie_mlp_train": ([vp, vp, vp, i64, i32, i32, f32, u32, vp, vp], i32),
     "fmm3dbie_mlp_predict": ([vp, vp, i64, vp, vp, vp], i32),

@@ -17,6 +17,7 @@
 #include "gmres.hpp"
 #include "host_basis.hpp"
 #include "learned.hpp"
+#include "near.hpp"
 #include "quad.hpp"
 
 namespace {
@@ -585,6 +586,66 @@ void lap_comb_dir_solver_guru_(const int64_t* npatches, const int64_t* norders, 
   solver_guru_host(false, *npatches, norders, ixyzs, iptype, *npts, srcvals, kp, *numit, *ifinout, rhs, *nnz,
                    row_ptr, col_ind, iquad, *nquad, wnear, novers, *nptso, ixyzso, srcover, whtsover, *eps_gmres,
                    niter, errs, rres, soln);
+}
+
+// ---------------------------------------------------------------- near-field finder
+// findnearmem / findnear (near_field_routs.f:193, :424) with host arrays
+void findnearmem_(const double* xyzs, const int64_t* ns, const double* rads, const int64_t* ndt,
+                  const double* targets, const int64_t* nt, int64_t* nnz) {
+  hipStream_t s = nullptr;
+  int rc = hipStreamCreate(&s) == hipSuccess ? 0 : -1;
+  double *dx = nullptr, *dr = nullptr, *dt = nullptr;
+  if (!rc) {
+    dx = upload(xyzs, 3 * *ns, s);
+    dr = upload(rads, *ns, s);
+    dt = upload(targets, *ndt * *nt, s);
+    if (!dx || !dr || !dt) rc = -2;
+  }
+  if (!rc) rc = findnear_count_device(*ns, dx, dr, *ndt, *nt, dt, nullptr, nnz, s);
+  void* bufs[] = {dx, dr, dt};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (s) (void)hipStreamDestroy(s);
+  g_last_error = rc;
+}
+
+void findnear_(const double* xyzs, const int64_t* ns, const double* rads, const int64_t* ndt, const double* targets,
+               const int64_t* nt, int64_t* row_ptr, int64_t* col_ind) {
+  hipStream_t s = nullptr;
+  int rc = hipStreamCreate(&s) == hipSuccess ? 0 : -1;
+  double *dx = nullptr, *dr = nullptr, *dt = nullptr;
+  int64_t *drp = nullptr, *dci = nullptr, nnz = 0;
+  if (!rc) {
+    dx = upload(xyzs, 3 * *ns, s);
+    dr = upload(rads, *ns, s);
+    dt = upload(targets, *ndt * *nt, s);
+    if (!dx || !dr || !dt) rc = -2;
+  }
+  if (!rc && hipMalloc((void**)&drp, sizeof(int64_t) * (*nt + 1)) != hipSuccess) rc = -2;
+  if (!rc) rc = findnear_count_device(*ns, dx, dr, *ndt, *nt, dt, drp, &nnz, s);
+  if (!rc && nnz > 0 && hipMalloc((void**)&dci, sizeof(int64_t) * nnz) != hipSuccess) rc = -2;
+  if (!rc && nnz > 0) rc = findnear_device(*ns, dx, dr, *ndt, *nt, dt, drp, dci, s);
+  if (!rc && hipMemcpy(row_ptr, drp, sizeof(int64_t) * (*nt + 1), hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+  if (!rc && nnz > 0 && hipMemcpy(col_ind, dci, sizeof(int64_t) * nnz, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+  void* bufs[] = {dx, dr, dt, drp, dci};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (s) (void)hipStreamDestroy(s);
+  g_last_error = rc;
+}
+
+int fmm3dbie_findnearmem_dev(int64_t ns, const double* xyzs, const double* rads, int64_t ndt, int64_t nt,
+                             const double* targets, int64_t* row_ptr, int64_t* nnz, void* stream) {
+  const int rc = findnear_count_device(ns, xyzs, rads, ndt, nt, targets, row_ptr, nnz, (hipStream_t)stream);
+  g_last_error = rc;
+  return rc;
+}
+
+int fmm3dbie_findnear_dev(int64_t ns, const double* xyzs, const double* rads, int64_t ndt, int64_t nt,
+                          const double* targets, int64_t* row_ptr, int64_t* col_ind, void* stream) {
+  const int rc = findnear_device(ns, xyzs, rads, ndt, nt, targets, row_ptr, col_ind, (hipStream_t)stream);
+  g_last_error = rc;
+  return rc;
 }
 
 // Free the device work buffers the near-quadrature calls keep between calls.

@@ -83,6 +83,30 @@ def findnear(cms, rads, targs):
     return row_ptr.astype(np.int64), (c + 1).astype(np.int64)
 
 
+def findnear_gpu(cms, rads, targs):
+    """findnear on the GPU through the findnearmem_/findnear_ drop-ins (same result as
+    findnear: 1-based CSR, ascending within a row)."""
+    import ctypes as C
+    from ._lib import lib
+    xyz = np.ascontiguousarray(cms, np.float64)
+    r = np.ascontiguousarray(rads, np.float64)
+    t = np.ascontiguousarray(targs, np.float64)
+    ns, nt, ndt = len(xyz), len(t), t.shape[1] if t.ndim == 2 else 3
+    i8 = lambda v: C.byref(C.c_int64(v))  # noqa: E731
+    p = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+    nnz = C.c_int64(0)
+    L = lib()
+    L.findnearmem_(p(xyz), i8(ns), p(r), i8(ndt), p(t), i8(nt), C.byref(nnz))
+    if L.fmm3dbie_last_error() != 0:
+        raise RuntimeError("findnearmem failed (status %d)" % L.fmm3dbie_last_error())
+    row_ptr = np.zeros(nt + 1, np.int64)
+    col_ind = np.zeros(max(nnz.value, 1), np.int64)
+    L.findnear_(p(xyz), i8(ns), p(r), i8(ndt), p(t), i8(nt), p(row_ptr), p(col_ind))
+    if L.fmm3dbie_last_error() != 0:
+        raise RuntimeError("findnear failed (status %d)" % L.fmm3dbie_last_error())
+    return row_ptr, col_ind[:nnz.value]
+
+
 def get_iquad_rsc(geom, row_ptr, col_ind):
     npols = np.diff(geom["ixyzs"])
     iquad = np.ones(len(col_ind) + 1, np.int64)

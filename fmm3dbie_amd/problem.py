@@ -8,14 +8,16 @@ import numpy as np
 from . import near
 
 
-def setup(geom, eps, helm, pars, far_order=True, nover_fixed=None):
+def setup(geom, eps, helm, pars, far_order=True, nover_fixed=None, gpu_near=False):
     """Returns a dict with row_ptr, col_ind, iquad (1-based), rfac, rfac0, cms, rads,
-    novers, ixyzso, srcover, whtsover, wts, ipatch_id, uvs_targ, nnz, nquad."""
+    novers, ixyzso, srcover, whtsover, wts, ipatch_id, uvs_targ, nnz, nquad.
+    gpu_near: build the near list with the GPU finder (same result)."""
     norder = near.norder_avg(geom)
     rfac, rfac0 = near.get_rfacs(norder)
     cms, rads = near.get_centroid_rads(geom)
     targs = geom["srcvals"][:, :3]
-    row_ptr, col_ind = near.findnear(cms, rads * rfac, targs)
+    finder = near.findnear_gpu if gpu_near else near.findnear
+    row_ptr, col_ind = finder(cms, rads * rfac, targs)
     iquad = near.get_iquad_rsc(geom, row_ptr, col_ind)
     if helm:
         zk = complex(np.asarray(pars, np.complex128)[0])

@@ -124,6 +124,14 @@ void get_iquad_rsc_(const int64_t *npatches, const int64_t *ixyzs, const int64_t
 /* src/quadratures/near_field_routs.f:998-1057  get_near_corr_max */
 void get_near_corr_max_(const int64_t *ntarg, const int64_t *row_ptr, const int64_t *nnz,
                         const int64_t *col_ind, const int64_t *npatches, const int64_t *ixyzs, int64_t *nmax);
+/* src/quadratures/near_field_routs.f:193-423  findnearmem: nnz of the near list
+ * (sources j with |x_i - c_j|^2 <= rads_j^2), computed on the GPU */
+void findnearmem_(const double *xyzs, const int64_t *ns, const double *rads, const int64_t *ndt,
+                  const double *targets, const int64_t *nt, int64_t *nnz);
+/* src/quadratures/near_field_routs.f:424-757  findnear: row_ptr (nt+1), col_ind (nnz),
+ * 1-based, ascending within a row (the order of findnearslow, :38-131) */
+void findnear_(const double *xyzs, const int64_t *ns, const double *rads, const int64_t *ndt,
+               const double *targets, const int64_t *nt, int64_t *row_ptr, int64_t *col_ind);
 /* src/common/get_fmm_thresh.f:1-90  get_fmm_thresh */
 void get_fmm_thresh_(const int64_t *nds, const int64_t *ns, const double *src, const int64_t *ndt,
                      const int64_t *nt, const double *trg, double *thresh);
@@ -170,6 +178,13 @@ int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatc
                           const int64_t *iquad, double rfac0, double *wnear, int self_n, double self_tau_panel,
                           void *stream, double *stats, int *nref_host, int *class_host, const float *mlp_params,
                           float *feat_out, int *pred_host);
+
+/* findnearmem / findnear on device arrays (xyzs (3,ns), rads (ns), targets (ndt,nt));
+ * row_ptr may be null for the count. */
+int fmm3dbie_findnearmem_dev(int64_t ns, const double *xyzs, const double *rads, int64_t ndt, int64_t nt,
+                             const double *targets, int64_t *row_ptr, int64_t *nnz, void *stream);
+int fmm3dbie_findnear_dev(int64_t ns, const double *xyzs, const double *rads, int64_t ndt, int64_t nt,
+                          const double *targets, int64_t *row_ptr, int64_t *col_ind, void *stream);
 
 /* Near-quadrature calls keep their device work buffers (grow-only, per device) and
  * per-order tables between calls and are serialised by a lock; this frees the buffers.
