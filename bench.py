@@ -108,6 +108,13 @@ def main():
     t0 = time.time()
     P = problem.setup(g, eps, helm, pars)
     t_setup_host = time.time() - t0
+    # the same setup with the near list and far orders computed on the GPU
+    problem.setup(g, eps, helm, pars, gpu_near=True)
+    t0 = time.time()
+    Pg = problem.setup(g, eps, helm, pars, gpu_near=True)
+    t_setup_gpu = time.time() - t0
+    setup_same = bool(np.array_equal(Pg["col_ind"], P["col_ind"]) and np.array_equal(Pg["novers"], P["novers"]))
+    del Pg
     N = len(g["srcvals"])
     p = int(np.diff(g["ixyzs"])[0])
     nq = layer.NearQuadDevice(g, helm, pars, eps, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], device=dev)
@@ -169,6 +176,8 @@ def main():
                      "kernel": "k_adaptive", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None},
         "setup_host_s": t_setup_host,
+        "setup_gpu_near_s": t_setup_gpu,
+        "setup_gpu_near_identical": setup_same,
     }
     tr = measured_traffic("k_adaptive", wname) if name == "adaptive" else None
     if tr is not None:

@@ -30,6 +30,9 @@ SIGS = {
     "findnearmem_": ([vp, vp, vp, vp, vp, vp, vp], None),
     "findnear_": ([vp, vp, vp, vp, vp, vp, vp, vp], None),
     "fmm3dbie_findnearmem_dev": ([i64, vp, vp, i64, i64, vp, vp, vp, vp], i32),
+    "get_far_order_": ([vp] * 20, None),
+    "fmm3dbie_far_order_dev": ([f64, i64, i32, vp, vp, vp, vp, i64, i64, vp, i32, vp, i64, vp, vp, f64, vp, vp],
+                               i32),
     "fmm3dbie_findnear_dev": ([i64, vp, vp, i64, i64, vp, vp, vp, vp], i32),
     "fmm3dbie_mlp_grad": ([vp, vp, vp, i64, vp, vp, vp], i32),
     "fmm3dbie_mlp_train": ([vp, vp, vp, i64, i32, i32, f32, u32, vp, vp], i32),

@@ -16,6 +16,7 @@
 #include "apply.hpp"
 #include "gmres.hpp"
 #include "host_basis.hpp"
+#include "far.hpp"
 #include "learned.hpp"
 #include "near.hpp"
 #include "quad.hpp"
@@ -644,6 +645,76 @@ int fmm3dbie_findnearmem_dev(int64_t ns, const double* xyzs, const double* rads,
 int fmm3dbie_findnear_dev(int64_t ns, const double* xyzs, const double* rads, int64_t ndt, int64_t nt,
                           const double* targets, int64_t* row_ptr, int64_t* col_ind, void* stream) {
   const int rc = findnear_device(ns, xyzs, rads, ndt, nt, targets, row_ptr, col_ind, (hipStream_t)stream);
+  g_last_error = rc;
+  return rc;
+}
+
+// ---------------------------------------------------------------- far-field order
+// get_far_order (far_field_routs.f90:10-171) with host arrays: nfars (npatches) and
+// ixyzso (npatches + 1) out
+void get_far_order_(const double* eps, const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                    const int64_t* iptype, const double* cms, const double* rads, const int64_t* npts,
+                    const double* srccoefs, const int64_t* ndtarg, const int64_t* ntarg, const double* targvals,
+                    const int64_t* ikerorder, const double* zk, const int64_t* nnz, const int64_t* row_ptr,
+                    const int64_t* col_ind, const double* rfac, int64_t* nfars, int64_t* ixyzso) {
+  const int64_t np = *npatches;
+  int rc = np > 0 ? uniform_order(np, norders, ixyzs, iptype) : 0;
+  hipStream_t s = nullptr;
+  if (rc >= 0 && hipStreamCreate(&s) != hipSuccess) rc = -1;
+  FarOrderArgs A = {};
+  std::vector<void*> bufs;
+  auto up = [&](auto* h, size_t n) {
+    auto* d = upload(h, n, s);
+    bufs.push_back((void*)d);
+    if (!d) rc = -2;
+    return d;
+  };
+  int64_t* dnov = nullptr;
+  if (rc >= 0 && np > 0) {
+    A.eps = *eps;
+    A.npatches = np;
+    A.norder = rc;
+    A.ixyzs = up(ixyzs, np + 1);
+    A.cms = up(cms, 3 * np);
+    A.rads = up(rads, np);
+    A.srccoefs = up(srccoefs, 9 * *npts);
+    A.ndtarg = *ndtarg;
+    A.ntarg = *ntarg;
+    A.targs = up(targvals, *ndtarg * *ntarg);
+    A.ikerorder = (int)*ikerorder;
+    A.zk[0] = zk[0];
+    A.zk[1] = zk[1];
+    A.nnz = *nnz;
+    A.row_ptr = up(row_ptr, *ntarg + 1);
+    A.col_ind = up(col_ind, *nnz);
+    A.rfac = *rfac;
+    if (rc >= 0 && hipMalloc((void**)&dnov, sizeof(int64_t) * np) != hipSuccess) rc = -2;
+    if (rc >= 0) {
+      A.novers = dnov;
+      rc = far_order_device(A, s);
+    }
+    if (rc >= 0 && hipMemcpy(nfars, dnov, sizeof(int64_t) * np, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+  }
+  if (rc >= 0) {
+    ixyzso[0] = 1;
+    for (int64_t i = 0; i < np; i++) ixyzso[i + 1] = ixyzso[i] + fmmhost::npols((int)nfars[i]);
+    rc = 0;
+  }
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (dnov) (void)hipFree(dnov);
+  if (s) (void)hipStreamDestroy(s);
+  g_last_error = rc;
+}
+
+int fmm3dbie_far_order_dev(double eps, int64_t npatches, int norder, const int64_t* ixyzs, const double* cms,
+                           const double* rads, const double* srccoefs, int64_t ndtarg, int64_t ntarg,
+                           const double* targs, int ikerorder, const double* zk, int64_t nnz,
+                           const int64_t* row_ptr, const int64_t* col_ind, double rfac, int64_t* novers,
+                           void* stream) {
+  FarOrderArgs A = {eps, npatches, norder, ixyzs, cms, rads, srccoefs, ndtarg, ntarg, targs, ikerorder,
+                    {zk[0], zk[1]}, nnz, row_ptr, col_ind, rfac, novers};
+  const int rc = far_order_device(A, (hipStream_t)stream);
   g_last_error = rc;
   return rc;
 }

@@ -221,4 +221,37 @@ void interp_matrix(int norder, int nover, std::vector<double>& imat) {
   }
 }
 
+// x_{k+1} = a x_k mod 2^48 with a = 33952834046453 held as four 12-bit limbs
+// (494, 322, 2508, 2549); the seed limbs are updated in place and the result is the
+// seed scaled to [0, 1), regenerated if it rounds to 1.
+double dlaran(int iseed[4]) {
+  static const int a[4] = {494, 322, 2508, 2549};
+  const int base = 4096;
+  const double r = 1.0 / base;
+  for (;;) {
+    // schoolbook product of the limb vectors, keeping the low 48 bits
+    int p3 = iseed[3] * a[3];
+    int c3 = p3 / base;
+    p3 -= base * c3;
+    int p2 = c3 + iseed[2] * a[3] + iseed[3] * a[2];
+    int c2 = p2 / base;
+    p2 -= base * c2;
+    int p1 = c2 + iseed[1] * a[3] + iseed[2] * a[2] + iseed[3] * a[1];
+    int c1 = p1 / base;
+    p1 -= base * c1;
+    int p0 = (c1 + iseed[0] * a[3] + iseed[1] * a[2] + iseed[2] * a[1] + iseed[3] * a[0]) % base;
+    iseed[0] = p0;
+    iseed[1] = p1;
+    iseed[2] = p2;
+    iseed[3] = p3;
+    const double v = r * ((double)p0 + r * ((double)p1 + r * ((double)p2 + r * (double)p3)));
+    if (v != 1.0) return v;
+  }
+}
+
+double hkrand(int seed) {
+  int s[4] = {0, 0, 0, (2 * seed + 1) % 4096};
+  return dlaran(s);
+}
+
 }  // namespace fmmhost

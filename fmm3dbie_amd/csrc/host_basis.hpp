@@ -31,5 +31,9 @@ void fill_koorn_const(KoornConst* kc);
 // Interpolation from order-norder RV values to order-nover RV nodes, row-major
 // (po x p): out[i*p + l].
 void interp_matrix(int norder, int nover, std::vector<double>& imat);
+// LAPACK's dlaran (48-bit multiplicative congruential generator, 12-bit limbs) and the
+// reference's hkrand wrapper for a nonzero seed (src/common/hkrand.f:1-19).
+double dlaran(int iseed[4]);
+double hkrand(int seed);
 
 }  // namespace fmmhost

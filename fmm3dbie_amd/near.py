@@ -107,6 +107,32 @@ def findnear_gpu(cms, rads, targs):
     return row_ptr, col_ind[:nnz.value]
 
 
+def get_far_order_gpu(eps, geom, cms, rads, targs, ikerorder, zk, row_ptr, col_ind, rfac):
+    """get_far_order on the GPU through the get_far_order_ drop-in (same result as
+    get_far_order); returns (novers, ixyzso)."""
+    import ctypes as C
+    from ._lib import lib
+    npat = len(geom["norders"])
+    t = np.ascontiguousarray(np.asarray(targs)[:, :3], np.float64)
+    arr = lambda a, dt: np.ascontiguousarray(a, dt)  # noqa: E731
+    keep = [arr(geom["norders"], np.int64), arr(geom["ixyzs"], np.int64), arr(geom.get("iptype", np.ones(npat)),
+                                                                          np.int64),
+            arr(cms, np.float64), arr(rads, np.float64), arr(geom["srccoefs"], np.float64), t,
+            arr(row_ptr, np.int64), arr(col_ind, np.int64), np.array([complex(zk).real, complex(zk).imag])]
+    p = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+    i8 = lambda v: C.byref(C.c_int64(v))  # noqa: E731
+    novers = np.zeros(npat, np.int64)
+    ixyzso = np.zeros(npat + 1, np.int64)
+    L = lib()
+    L.get_far_order_(C.byref(C.c_double(eps)), i8(npat), p(keep[0]), p(keep[1]), p(keep[2]), p(keep[3]), p(keep[4]),
+                     i8(len(geom["srccoefs"])), p(keep[5]), i8(3), i8(len(t)), p(keep[6]), i8(int(ikerorder)),
+                     p(keep[9]), i8(len(keep[8])), p(keep[7]), p(keep[8]), C.byref(C.c_double(rfac)), p(novers),
+                     p(ixyzso))
+    if L.fmm3dbie_last_error() != 0:
+        raise RuntimeError("get_far_order failed (status %d)" % L.fmm3dbie_last_error())
+    return novers, ixyzso
+
+
 def get_iquad_rsc(geom, row_ptr, col_ind):
     npols = np.diff(geom["ixyzs"])
     iquad = np.ones(len(col_ind) + 1, np.int64)

@@ -11,7 +11,7 @@ from . import near
 def setup(geom, eps, helm, pars, far_order=True, nover_fixed=None, gpu_near=False):
     """Returns a dict with row_ptr, col_ind, iquad (1-based), rfac, rfac0, cms, rads,
     novers, ixyzso, srcover, whtsover, wts, ipatch_id, uvs_targ, nnz, nquad.
-    gpu_near: build the near list with the GPU finder (same result)."""
+    gpu_near: build the near list and the far-field orders on the GPU (same result)."""
     norder = near.norder_avg(geom)
     rfac, rfac0 = near.get_rfacs(norder)
     cms, rads = near.get_centroid_rads(geom)
@@ -29,8 +29,8 @@ def setup(geom, eps, helm, pars, far_order=True, nover_fixed=None, gpu_near=Fals
     if nover_fixed is not None:
         novers = np.full(len(geom["norders"]), int(nover_fixed), np.int64)
     elif far_order:
-        novers, _ = near.get_far_order(eps, geom, cms, rads, targs, ikerorder, zk.real if helm else 0.0, row_ptr,
-                                       col_ind, rfac)
+        far = near.get_far_order_gpu if gpu_near else near.get_far_order
+        novers, _ = far(eps, geom, cms, rads, targs, ikerorder, zk.real if helm else 0.0, row_ptr, col_ind, rfac)
     else:
         novers = np.asarray(geom["norders"], np.int64).copy()
     srcover, whtsover, ixyzso = near.oversample_geom(geom, novers)

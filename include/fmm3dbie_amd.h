@@ -132,6 +132,13 @@ void findnearmem_(const double *xyzs, const int64_t *ns, const double *rads, con
  * 1-based, ascending within a row (the order of findnearslow, :38-131) */
 void findnear_(const double *xyzs, const int64_t *ns, const double *rads, const int64_t *ndt,
                const double *targets, const int64_t *nt, int64_t *row_ptr, int64_t *col_ind);
+/* src/quadratures/far_field_routs.f90:10-171  get_far_order (computed on the GPU):
+ * oversampling order per patch and ixyzso; zk complex (2 doubles); patch orders 1..8 */
+void get_far_order_(const double *eps, const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs,
+                    const int64_t *iptype, const double *cms, const double *rads, const int64_t *npts,
+                    const double *srccoefs, const int64_t *ndtarg, const int64_t *ntarg, const double *targvals,
+                    const int64_t *ikerorder, const double *zk, const int64_t *nnz, const int64_t *row_ptr,
+                    const int64_t *col_ind, const double *rfac, int64_t *nfars, int64_t *ixyzso);
 /* src/common/get_fmm_thresh.f:1-90  get_fmm_thresh */
 void get_fmm_thresh_(const int64_t *nds, const int64_t *ns, const double *src, const int64_t *ndt,
                      const int64_t *nt, const double *trg, double *thresh);
@@ -185,6 +192,13 @@ int fmm3dbie_findnearmem_dev(int64_t ns, const double *xyzs, const double *rads,
                              const double *targets, int64_t *row_ptr, int64_t *nnz, void *stream);
 int fmm3dbie_findnear_dev(int64_t ns, const double *xyzs, const double *rads, int64_t ndt, int64_t nt,
                           const double *targets, int64_t *row_ptr, int64_t *col_ind, void *stream);
+
+/* get_far_order on device arrays (uniform patch order norder); novers out (npatches) */
+int fmm3dbie_far_order_dev(double eps, int64_t npatches, int norder, const int64_t *ixyzs, const double *cms,
+                           const double *rads, const double *srccoefs, int64_t ndtarg, int64_t ntarg,
+                           const double *targs, int ikerorder, const double *zk, int64_t nnz,
+                           const int64_t *row_ptr, const int64_t *col_ind, double rfac, int64_t *novers,
+                           void *stream);
 
 /* Near-quadrature calls keep their device work buffers (grow-only, per device) and
  * per-order tables between calls and are serialised by a lock; this frees the buffers.
