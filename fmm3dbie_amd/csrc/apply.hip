@@ -389,12 +389,15 @@ int apply_plan_build(ApplyPlan* P, const ApplyPlanInput& in, hipStream_t s) {
   P->npatches = in.npatches;
   P->npts = in.npts;
   P->nptso = in.nptso;
+  P->ntarg = in.targs ? in.ntarg : in.npts;
   P->nnz = in.nnz;
   P->helm = in.helm;
   P->zk[0] = in.zk[0]; P->zk[1] = in.zk[1];
   P->alpha[0] = in.alpha[0]; P->alpha[1] = in.alpha[1];
   P->beta[0] = in.beta[0]; P->beta[1] = in.beta[1];
-  const int64_t npat = in.npatches, N = in.npts, NO = in.nptso;
+  const int64_t npat = in.npatches, N = P->ntarg, NO = in.nptso;
+  const double* tsrc = in.targs ? in.targs : in.srcvals;  // target coordinates
+  const int64_t tstride = in.targs ? in.ndtarg : 12;
   // host-side tables: interpolation matrices per (norder, nover), patch of each
   // oversampled point, sorted near lists, uniform-p check
   std::vector<int64_t> ioff(npat);
@@ -442,7 +445,7 @@ int apply_plan_build(ApplyPlan* P, const ApplyPlanInput& in, hipStream_t s) {
     }
   for (int64_t i = 0; i < N; i++)
     for (int m = 0; m < 3; m++) {
-      const double v = in.srcvals[12 * i + m];
+      const double v = tsrc[tstride * i + m];
       mn[m] = v < mn[m] ? v : mn[m];
       mx[m] = v > mx[m] ? v : mx[m];
     }
@@ -479,7 +482,7 @@ int apply_plan_build(ApplyPlan* P, const ApplyPlanInput& in, hipStream_t s) {
   P->wnear64 = own<double>(P, 2 * nq);
   P->wnear32 = own<float>(P, 2 * nq);
   double* d_srcover = dal<double>(12 * NO);
-  double* d_srcvals = dal<double>(12 * N);
+  double* d_srcvals = dal<double>(tstride * N);
   if (!d_srcover || !d_srcvals || !P->wnear32 || !P->wnear64) return -2;
   ACHK(hipMemcpyAsync(P->ixyzs, in.ixyzs, 8 * (npat + 1), hipMemcpyHostToDevice, s));
   ACHK(hipMemcpyAsync(P->ixyzso, in.ixyzso, 8 * (npat + 1), hipMemcpyHostToDevice, s));
@@ -494,8 +497,10 @@ int apply_plan_build(ApplyPlan* P, const ApplyPlanInput& in, hipStream_t s) {
   ACHK(hipMemcpyAsync(P->imat, imat_all.data(), 8 * imat_all.size(), hipMemcpyHostToDevice, s));
   ACHK(hipMemcpyAsync(P->wover, in.whtsover, 8 * NO, hipMemcpyHostToDevice, s));
   ACHK(hipMemcpyAsync(d_srcover, in.srcover, 8 * 12 * NO, hipMemcpyHostToDevice, s));
-  ACHK(hipMemcpyAsync(d_srcvals, in.srcvals, 8 * 12 * N, hipMemcpyHostToDevice, s));
-  if (in.wnear_is_device) {
+  if (N > 0) ACHK(hipMemcpyAsync(d_srcvals, tsrc, 8 * tstride * N, hipMemcpyHostToDevice, s));
+  if (nq == 0) {
+    // no near pairs: nothing to copy
+  } else if (in.wnear_is_device) {
     if (in.helm) {
       ACHK(hipMemcpyAsync(P->wnear64, in.wnear, 16 * nq, hipMemcpyDeviceToDevice, s));
     } else {
@@ -511,14 +516,18 @@ int apply_plan_build(ApplyPlan* P, const ApplyPlanInput& in, hipStream_t s) {
     ACHK(hipStreamSynchronize(s));
     (void)hipFree(tmp);
   }
-  k_convert<double, float><<<nblk(2 * nq, 256), 256, 0, s>>>(2 * nq, P->wnear64, P->wnear32);
-  k_split_src<<<nblk(NO, 256), 256, 0, s>>>(NO, d_srcover, P->src64[0], P->src64[1], P->src64[2], P->src64[3],
-                                            P->src64[4], P->src64[5], 12);
-  k_split_src<<<nblk(N, 256), 256, 0, s>>>(N, d_srcvals, P->trg64[0], P->trg64[1], P->trg64[2], nullptr, nullptr,
-                                          nullptr, 12);
-  for (int c = 0; c < 6; c++) k_convert<double, float><<<nblk(NO, 256), 256, 0, s>>>(NO, P->src64[c], P->src32[c]);
-  for (int c = 0; c < 3; c++) k_convert<double, float><<<nblk(N, 256), 256, 0, s>>>(N, P->trg64[c], P->trg32[c]);
-  k_convert<double, float><<<nblk(NO, 256), 256, 0, s>>>(NO, P->wover, P->wover32);
+  if (nq > 0) k_convert<double, float><<<nblk(2 * nq, 256), 256, 0, s>>>(2 * nq, P->wnear64, P->wnear32);
+  if (NO > 0)
+    k_split_src<<<nblk(NO, 256), 256, 0, s>>>(NO, d_srcover, P->src64[0], P->src64[1], P->src64[2], P->src64[3],
+                                              P->src64[4], P->src64[5], 12);
+  if (N > 0)
+    k_split_src<<<nblk(N, 256), 256, 0, s>>>(N, d_srcvals, P->trg64[0], P->trg64[1], P->trg64[2], nullptr, nullptr,
+                                            nullptr, tstride);
+  if (NO > 0)
+    for (int c = 0; c < 6; c++) k_convert<double, float><<<nblk(NO, 256), 256, 0, s>>>(NO, P->src64[c], P->src32[c]);
+  if (N > 0)
+    for (int c = 0; c < 3; c++) k_convert<double, float><<<nblk(N, 256), 256, 0, s>>>(N, P->trg64[c], P->trg32[c]);
+  if (NO > 0) k_convert<double, float><<<nblk(NO, 256), 256, 0, s>>>(NO, P->wover, P->wover32);
   ACHK(hipGetLastError());
   ACHK(hipStreamSynchronize(s));
   (void)hipFree(d_srcover);
@@ -539,9 +548,9 @@ static int ensure_ws(ApplyPlan* P, int nrhs) {
   P->q32 = dal<float>(2 * NO * nrhs);
   P->m32 = dal<float>(2 * NO * nrhs);
   P->sig64c = dal<double>(2 * (size_t)P->npts);
-  P->pot64c = dal<double>(2 * (size_t)P->npts);
+  P->pot64c = dal<double>(2 * (size_t)P->ntarg);
   // partial far sums: nsplit x nrhs x npts with nsplit x nrhs <= far_split_cap x 64
-  const size_t npart = (size_t)P->far_split_cap * 64 * P->npts;
+  const size_t npart = (size_t)P->far_split_cap * 64 * P->ntarg;
   P->part64 = dal<double>(2 * npart);
   P->part32 = dal<float>(2 * npart);
   if (!P->q64 || !P->m64 || !P->q32 || !P->m32 || !P->sig64c || !P->pot64c || !P->part64 || !P->part32) return -2;
@@ -576,7 +585,7 @@ static int launch_far(ApplyPlan* P, bool mask, int64_t wg_target, const V2<T>* q
   }
   const T kr = (T)(P->helm ? P->zk[0] : 0.0), ki = (T)(P->helm ? P->zk[1] : 0.0);
   const T th2 = (T)(P->thresh * P->thresh);
-  const unsigned g = nblk(P->npts, TPB);
+  const unsigned g = nblk(P->ntarg, TPB);
   const int ngrp = nrhs / NRHS;
   const int64_t ntiles = (P->nptso + TPB - 1) / TPB;
   int64_t nsplit = (wg_target + (int64_t)g * ngrp - 1) / ((int64_t)g * ngrp);
@@ -586,14 +595,14 @@ static int launch_far(ApplyPlan* P, bool mask, int64_t wg_target, const V2<T>* q
   if (nsplit < 1) nsplit = 1;
   const int64_t chunk = ((ntiles + nsplit - 1) / nsplit) * TPB;
   nsplit = (P->nptso + chunk - 1) / chunk;
-  const int64_t ldpart = (int64_t)nrhs * P->npts;
+  const int64_t ldpart = (int64_t)nrhs * P->ntarg;
   dim3 grid(g, (unsigned)nsplit, (unsigned)ngrp);
   if (mask)
-    k_farsum<T, true, NRHS><<<grid, TPB, 0, s>>>(P->npts, trg[0], trg[1], trg[2], P->nptso, src[0], src[1], src[2],
+    k_farsum<T, true, NRHS><<<grid, TPB, 0, s>>>(P->ntarg, trg[0], trg[1], trg[2], P->nptso, src[0], src[1], src[2],
                                                  src[3], src[4], src[5], P->opatch, q, m, ldstr, P->row_ptr,
                                                  P->near_sorted, kr, ki, th2, chunk, part, ldpart);
   else
-    k_farsum<T, false, NRHS><<<grid, TPB, 0, s>>>(P->npts, trg[0], trg[1], trg[2], P->nptso, src[0], src[1],
+    k_farsum<T, false, NRHS><<<grid, TPB, 0, s>>>(P->ntarg, trg[0], trg[1], trg[2], P->nptso, src[0], src[1],
                                                   src[2], src[3], src[4], src[5], P->opatch, q, m, ldstr,
                                                   P->row_ptr, P->near_sorted, kr, ki, th2, chunk, part, ldpart);
   *nsplit_out = nsplit;
@@ -633,7 +642,7 @@ static int dispatch_far(ApplyPlan* P, bool mask, int grp, int64_t wg, const V2<T
 template <typename T>
 static int far_and_csr(ApplyPlan* P, int prec, bool mask, const V2<T>* q, const V2<T>* m, const V2<T>* sg,
                        V2<T>* pt, int nrhs, hipStream_t s, hipEvent_t ev_far) {
-  const int64_t N = P->npts, NO = P->nptso;
+  const int64_t N = P->ntarg, NO = P->nptso, NS = P->npts;
   V2<T>* part = (V2<T>*)(sizeof(T) == 8 ? (void*)P->part64 : (void*)P->part32);
   const V2<T>* wn = (const V2<T>*)(sizeof(T) == 8 ? (void*)P->wnear64 : (void*)P->wnear32);
   int64_t nsplit = 0;
@@ -650,11 +659,11 @@ static int far_and_csr(ApplyPlan* P, int prec, bool mask, const V2<T>* q, const 
   if (nrhs == 1)                                                                                               \
     k_reduce_csr<T, T, 1, PC><<<nblk(N, 4), 256, 0, s>>>(N, nrhs, (int)nsplit, part, (int64_t)nrhs * N,         \
                                                          P->row_ptr, P->csr_src, P->iquad, P->uniform_p, wn, sg, \
-                                                         N, pt, N);                                              \
+                                                         NS, pt, N);                                             \
   else                                                                                                         \
     k_reduce_csr<T, T, 8, PC><<<nblk(N, 4), 256, 0, s>>>(N, nrhs, (int)nsplit, part, (int64_t)nrhs * N,         \
                                                          P->row_ptr, P->csr_src, P->iquad, P->uniform_p, wn, sg, \
-                                                         N, pt, N)
+                                                         NS, pt, N)
     switch (P->uniform_p) {
       case 3: FMM_CSR(3); break;
       case 6: FMM_CSR(6); break;
@@ -675,7 +684,7 @@ static int far_and_csr(ApplyPlan* P, int prec, bool mask, const V2<T>* q, const 
                                                                   N);
     else
       k_fill<T><<<nblk(2 * N * nrhs, 256), 256, 0, s>>>(2 * N * nrhs, (T*)pt, (T)0);
-    k_csr_gen<T, T><<<nblk(N, 256), 256, 0, s>>>(N, P->row_ptr, P->col_ind, P->iquad, P->ixyzs, wn, sg, N, nrhs,
+    k_csr_gen<T, T><<<nblk(N, 256), 256, 0, s>>>(N, P->row_ptr, P->col_ind, P->iquad, P->ixyzs, wn, sg, NS, nrhs,
                                                  pt, N);
   }
   ACHK(hipGetLastError());
@@ -687,7 +696,7 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
                    ApplyTimes* tm) {
   if (nrhs < 1 || nrhs > 64) return -6;
   if (ensure_ws(P, nrhs) != 0) return -2;
-  const int64_t N = P->npts, NO = P->nptso;
+  const int64_t N = P->npts, NT = P->ntarg, NO = P->nptso;
   const double ar = P->alpha[0], ai = P->helm ? P->alpha[1] : 0.0;
   const double br = P->beta[0], bi = P->helm ? P->beta[1] : 0.0;
   hipEvent_t ev[5];
@@ -698,7 +707,8 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
   if (prec == 64) {
     const V2<double>* sg = (const V2<double>*)sigma;
     V2<double>* pt = (V2<double>*)pot;
-    k_oversample<double><<<nblk(NO, 256), 256, 0, s>>>(NO, P->opatch, P->ixyzs, P->ixyzso, P->ioff, P->np_of, P->imat,
+    if (NO > 0)
+      k_oversample<double><<<nblk(NO, 256), 256, 0, s>>>(NO, P->opatch, P->ixyzs, P->ixyzso, P->ioff, P->np_of, P->imat,
                                                        sg, nrhs, N, P->wover, ar, ai, br, bi,
                                                        (V2<double>*)P->q64, (V2<double>*)P->m64, NO);
     if (tm) ACHK(hipEventRecord(ev[1], s));
@@ -709,15 +719,18 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
     if (tm) ACHK(hipEventRecord(ev[3], s));
     if (!rc && !mask) {
       const double kr = P->helm ? P->zk[0] : 0.0, ki = P->helm ? P->zk[1] : 0.0;
-      k_near_sub<<<nblk(N, 4), 256, 0, s>>>(N, P->trg64[0], P->trg64[1], P->trg64[2], P->src64[0], P->src64[1],
-                                            P->src64[2], P->src64[3], P->src64[4], P->src64[5], q, m, NO, nrhs,
-                                            P->row_ptr, P->col_ind, P->ixyzso, kr, ki, P->thresh * P->thresh, pt, N);
+      if (NT > 0)
+        k_near_sub<<<nblk(NT, 4), 256, 0, s>>>(NT, P->trg64[0], P->trg64[1], P->trg64[2], P->src64[0], P->src64[1],
+                                               P->src64[2], P->src64[3], P->src64[4], P->src64[5], q, m, NO, nrhs,
+                                               P->row_ptr, P->col_ind, P->ixyzso, kr, ki, P->thresh * P->thresh, pt,
+                                               NT);
     }
     if (tm) ACHK(hipEventRecord(ev[4], s));
   } else {
     const V2<float>* sg = (const V2<float>*)sigma;
     V2<float>* pt = (V2<float>*)pot;
-    k_oversample<float><<<nblk(NO, 256), 256, 0, s>>>(NO, P->opatch, P->ixyzs, P->ixyzso, P->ioff, P->np_of, P->imat,
+    if (NO > 0)
+      k_oversample<float><<<nblk(NO, 256), 256, 0, s>>>(NO, P->opatch, P->ixyzs, P->ixyzso, P->ioff, P->np_of, P->imat,
                                                       sg, nrhs, N, P->wover32, ar, ai, br, bi, (V2<float>*)P->q32,
                                                       (V2<float>*)P->m32, NO);
     if (tm) ACHK(hipEventRecord(ev[1], s));
@@ -747,7 +760,7 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
 int apply_plan_tune(ApplyPlan* P, int prec, int nrhs, hipStream_t s, double* report, int nreport) {
   if (nrhs < 1 || nrhs > 64 || (prec != 32 && prec != 64)) return -6;
   if (ensure_ws(P, nrhs) != 0) return -2;
-  if (P->npts == 0 || P->nptso == 0) return 0;
+  if (P->ntarg == 0 || P->nptso == 0) return 0;
   const int64_t NO = P->nptso;
   if (prec == 64) {
     k_fill<double><<<nblk(2 * NO * nrhs, 256), 256, 0, s>>>(2 * NO * nrhs, P->q64, 1.0);
@@ -805,11 +818,11 @@ int apply_plan_tune(ApplyPlan* P, int prec, int nrhs, hipStream_t s, double* rep
 // real (Laplace) densities: convert to complex, apply, take real part
 int apply_plan_run_real(ApplyPlan* P, int mode, const double* sigma, double* pot, hipStream_t s) {
   if (ensure_ws(P, 1) != 0) return -2;
-  const int64_t N = P->npts;
+  const int64_t N = P->npts, NT = P->ntarg;
   k_real_to_cplx<<<nblk(N, 256), 256, 0, s>>>(N, sigma, (V2<double>*)P->sig64c);
   int r = apply_plan_run(P, 64, mode, P->sig64c, P->pot64c, 1, s, nullptr);
   if (r) return r;
-  k_cplx_to_real<<<nblk(N, 256), 256, 0, s>>>(N, (const V2<double>*)P->pot64c, pot);
+  if (NT > 0) k_cplx_to_real<<<nblk(NT, 256), 256, 0, s>>>(NT, (const V2<double>*)P->pot64c, pot);
   ACHK(hipGetLastError());
   return 0;
 }

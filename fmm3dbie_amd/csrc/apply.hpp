@@ -17,6 +17,9 @@ struct ApplyPlanInput {
   const int64_t *row_ptr, *col_ind, *iquad;          // host, 1-based
   const double *srcvals, *srcover, *whtsover;        // host
   const double* wnear;    // complex (interleaved) if helm, else real
+  // targets (host, (ndtarg, ntarg)); null: the discretization points (srcvals)
+  const double* targs = nullptr;
+  int64_t ndtarg = 12, ntarg = -1;
   bool wnear_is_device;   // wnear already in device memory
   double zk[2], alpha[2], beta[2];
 };
@@ -28,6 +31,7 @@ struct ApplyTimes {
 struct ApplyPlan {
   bool helm = true;
   int64_t npatches = 0, npts = 0, nptso = 0, nnz = 0, nquad = 0;
+  int64_t ntarg = 0;  // targets (rows of the near list); npts for the on-surface operator
   double zk[2] = {0, 0}, alpha[2] = {0, 0}, beta[2] = {0, 0};
   double thresh = 0;
   int uniform_p = 0;

@@ -134,7 +134,8 @@ int lpcomp_host(bool helm, int64_t npatches, const int64_t* norders, const int64
                 int64_t npts, const double* srcvals, const double* kpar6, int64_t nnz, const int64_t* row_ptr,
                 const int64_t* col_ind, const int64_t* iquad, int64_t nquad, const double* wnear,
                 const int64_t* novers, int64_t nptso, const int64_t* ixyzso, const double* srcover,
-                const double* whtsover, const double* sigma, double* pot) {
+                const double* whtsover, const double* sigma, double* pot, int64_t ndtarg = 12,
+                int64_t ntarg = -1, const double* targs = nullptr) {
   for (int64_t i = 0; i < npatches; i++)
     if (iptype && iptype[i] != 1) {
       fprintf(stderr, "fmm3dbie_amd: only triangular RV patches (iptype=1) are supported\n");
@@ -152,19 +153,23 @@ int lpcomp_host(bool helm, int64_t npatches, const int64_t* norders, const int64
   in.zk[0] = kpar6[0]; in.zk[1] = kpar6[1];
   in.alpha[0] = kpar6[2]; in.alpha[1] = kpar6[3];
   in.beta[0] = kpar6[4]; in.beta[1] = kpar6[5];
+  in.targs = targs;
+  in.ndtarg = ndtarg;
+  in.ntarg = targs ? ntarg : npts;
   int rc = 0;
   {
     ApplyPlan P;
     rc = apply_plan_build(&P, in, s);
-    const size_t n = (helm ? 2 : 1) * (size_t)npts;
+    const size_t n = (helm ? 2 : 1) * (size_t)npts, nt = (helm ? 2 : 1) * (size_t)in.ntarg;
     double *ds = nullptr, *dp = nullptr;
-    if (rc == 0 && (hipMalloc((void**)&ds, n * 8) != hipSuccess || hipMalloc((void**)&dp, n * 8) != hipSuccess))
+    if (rc == 0 && (hipMalloc((void**)&ds, (n ? n : 1) * 8) != hipSuccess ||
+                    hipMalloc((void**)&dp, (nt ? nt : 1) * 8) != hipSuccess))
       rc = -2;
-    if (rc == 0 && hipMemcpyAsync(ds, sigma, n * 8, hipMemcpyHostToDevice, s) != hipSuccess) rc = -1;
+    if (rc == 0 && n && hipMemcpyAsync(ds, sigma, n * 8, hipMemcpyHostToDevice, s) != hipSuccess) rc = -1;
     if (rc == 0)
       rc = helm ? apply_plan_run(&P, 64, APPLY_ADDSUB, ds, dp, 1, s, nullptr)
                 : apply_plan_run_real(&P, APPLY_ADDSUB, ds, dp, s);
-    if (rc == 0 && hipMemcpyAsync(pot, dp, n * 8, hipMemcpyDeviceToHost, s) != hipSuccess) rc = -1;
+    if (rc == 0 && nt && hipMemcpyAsync(pot, dp, nt * 8, hipMemcpyDeviceToHost, s) != hipSuccess) rc = -1;
     if (hipStreamSynchronize(s) != hipSuccess && rc == 0) rc = -1;
     if (ds) (void)hipFree(ds);
     if (dp) (void)hipFree(dp);
@@ -175,6 +180,173 @@ int lpcomp_host(bool helm, int64_t npatches, const int64_t* norders, const int64
 }
 
 }  // namespace
+
+// ---- host setup of the near/far split for arbitrary targets (the steps the reference's
+// helm_comb_dir_eval / _solver run before the quadrature, helm_comb_dir.f:536-600)
+
+// get_centroid_rads (surf_routs.f90:195-302): centroid of the three vertex images and the
+// largest vertex distance
+void centroid_rads(int64_t npatches, const int64_t* norders, const int64_t* ixyzs, const double* srccoefs,
+                   std::vector<double>& cms, std::vector<double>& rads) {
+  cms.assign(3 * npatches, 0.0);
+  rads.assign(npatches, 0.0);
+  std::vector<double> P(300);
+  const double vu[3] = {0.0, 1.0, 0.0}, vv[3] = {0.0, 0.0, 1.0};
+  for (int64_t ip = 0; ip < npatches; ip++) {
+    const int n = (int)norders[ip], p = fmmhost::npols(n);
+    const double* sc = srccoefs + 9 * (ixyzs[ip] - 1);
+    double xyz[3][3];
+    for (int k = 0; k < 3; k++) {
+      fmmhost::koorn_pols(vu[k], vv[k], n, P.data());
+      for (int m = 0; m < 3; m++) {
+        double acc = 0.0;
+        for (int l = 0; l < p; l++) acc += P[l] * sc[9 * l + m];
+        xyz[k][m] = acc;
+      }
+    }
+    for (int m = 0; m < 3; m++) cms[3 * ip + m] = (xyz[0][m] + xyz[1][m] + xyz[2][m]) / 3;
+    double r2 = 0.0;
+    for (int k = 0; k < 3; k++) {
+      double d = 0.0;
+      for (int m = 0; m < 3; m++) d += (xyz[k][m] - cms[3 * ip + m]) * (xyz[k][m] - cms[3 * ip + m]);
+      r2 = std::max(r2, d);
+    }
+    rads[ip] = std::sqrt(r2);
+  }
+}
+
+// oversample_geom + get_qwts (surf_routs.f90:304-449, 1598-1651): geometry at the RV
+// nodes of order novers[i] (the patch's own values when the orders agree) and weights
+// |x_u x x_v| w_RV
+void oversample_geom(int64_t npatches, const int64_t* norders, const int64_t* ixyzs, const double* srccoefs,
+                     const double* srcvals, const std::vector<int64_t>& novers, std::vector<int64_t>& ixyzso,
+                     std::vector<double>& srcover, std::vector<double>& wover) {
+  ixyzso.assign(npatches + 1, 1);
+  for (int64_t i = 0; i < npatches; i++) ixyzso[i + 1] = ixyzso[i] + fmmhost::npols((int)novers[i]);
+  const int64_t nso = ixyzso[npatches] - 1;
+  srcover.assign(12 * nso, 0.0);
+  wover.assign(nso, 0.0);
+  std::vector<double> P(300), uv(600), w(300);
+  for (int64_t ip = 0; ip < npatches; ip++) {
+    const int no = (int)norders[ip], nv = (int)novers[ip], p = fmmhost::npols(no), po = fmmhost::npols(nv);
+    fmmhost::rv_nodes(nv, uv.data(), w.data());
+    const double* sc = srccoefs + 9 * (ixyzs[ip] - 1);
+    double* out = srcover.data() + 12 * (ixyzso[ip] - 1);
+    for (int q = 0; q < po; q++) {
+      double* o = out + 12 * q;
+      if (nv == no) {
+        for (int m = 0; m < 12; m++) o[m] = srcvals[12 * (ixyzs[ip] - 1 + q) + m];
+      } else {
+        fmmhost::koorn_pols(uv[2 * q], uv[2 * q + 1], no, P.data());
+        for (int m = 0; m < 9; m++) {
+          double acc = 0.0;
+          for (int l = 0; l < p; l++) acc += P[l] * sc[9 * l + m];
+          o[m] = acc;
+        }
+        const double nx = o[4] * o[8] - o[5] * o[7], ny = o[5] * o[6] - o[3] * o[8], nz = o[3] * o[7] - o[4] * o[6];
+        const double r = std::sqrt(nx * nx + ny * ny + nz * nz);
+        o[9] = nx / r;
+        o[10] = ny / r;
+        o[11] = nz / r;
+      }
+      const double nx = o[4] * o[8] - o[5] * o[7], ny = o[5] * o[6] - o[3] * o[8], nz = o[3] * o[7] - o[4] * o[6];
+      wover[ixyzso[ip] - 1 + q] = std::sqrt(nx * nx + ny * ny + nz * nz) * w[q];
+    }
+  }
+}
+
+struct NearSetup {
+  double rfac = 0, rfac0 = 0;
+  std::vector<int64_t> row_ptr, col_ind, iquad, novers, ixyzso;
+  std::vector<double> srcover, wover;
+  int64_t nnz = 0, nquad = 0, nptso = 0;
+};
+
+// near list of the targets (GPU), iquad, far-field orders (GPU) and oversampled geometry
+int near_setup(int64_t npatches, const int64_t* norders, const int64_t* ixyzs, int64_t npts,
+               const double* srccoefs, const double* srcvals, int64_t ndtarg, int64_t ntarg, const double* targs,
+               double eps, const double* zk, double beta_abs, NearSetup& S) {
+  const int no = (int)norders[0];
+  fmmhost::rfacs(norder_avg(npatches, norders), &S.rfac, &S.rfac0);
+  std::vector<double> cms, rads;
+  centroid_rads(npatches, norders, ixyzs, srccoefs, cms, rads);
+  std::vector<double> rnear(npatches);
+  for (int64_t i = 0; i < npatches; i++) rnear[i] = rads[i] * S.rfac;
+  hipStream_t s = nullptr;
+  if (hipStreamCreate(&s) != hipSuccess) return -1;
+  std::vector<void*> bufs;
+  int rc = 0;
+  auto up = [&](auto* h, size_t n) {
+    auto* d = upload(h, n, s);
+    bufs.push_back((void*)d);
+    if (!d) rc = -2;
+    return d;
+  };
+  const double* dc = up(cms.data(), 3 * npatches);
+  const double* dr = up(rnear.data(), npatches);
+  const double* drad = up(rads.data(), npatches);
+  const double* dt = up(targs, ndtarg * ntarg);
+  const double* dsc = up(srccoefs, 9 * npts);
+  const int64_t* dix = up(ixyzs, npatches + 1);
+  int64_t *drp = nullptr, *dci = nullptr, *dnov = nullptr;
+  if (!rc && hipMalloc((void**)&drp, sizeof(int64_t) * (ntarg + 1)) != hipSuccess) rc = -2;
+  bufs.push_back(drp);
+  if (!rc) rc = findnear_count_device(npatches, dc, dr, ndtarg, ntarg, dt, drp, &S.nnz, s);
+  if (!rc && hipMalloc((void**)&dci, sizeof(int64_t) * (S.nnz ? S.nnz : 1)) != hipSuccess) rc = -2;
+  bufs.push_back(dci);
+  if (!rc && S.nnz > 0) rc = findnear_device(npatches, dc, dr, ndtarg, ntarg, dt, drp, dci, s);
+  S.row_ptr.assign(ntarg + 1, 1);
+  S.col_ind.assign(S.nnz, 0);
+  if (!rc && hipMemcpy(S.row_ptr.data(), drp, sizeof(int64_t) * (ntarg + 1), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = -1;
+  if (!rc && S.nnz && hipMemcpy(S.col_ind.data(), dci, sizeof(int64_t) * S.nnz, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = -1;
+  // far-field orders: ikerorder -1 (single layer) unless the double layer is present
+  S.novers.assign(npatches, no);
+  if (!rc && hipMalloc((void**)&dnov, sizeof(int64_t) * npatches) != hipSuccess) rc = -2;
+  bufs.push_back(dnov);
+  if (!rc) {
+    FarOrderArgs F = {eps, npatches, no, dix, dc, drad, dsc, ndtarg, ntarg, dt, beta_abs > 1e-16 ? 0 : -1,
+                      {zk[0], zk[1]}, S.nnz, drp, dci, S.rfac, dnov};
+    rc = far_order_device(F, s);
+  }
+  if (!rc && hipMemcpy(S.novers.data(), dnov, sizeof(int64_t) * npatches, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = -1;
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  (void)hipStreamDestroy(s);
+  if (rc) return rc;
+  S.iquad.assign(S.nnz + 1, 1);
+  for (int64_t j = 0; j < S.nnz; j++) S.iquad[j + 1] = S.iquad[j] + fmmhost::npols((int)norders[S.col_ind[j] - 1]);
+  S.nquad = S.iquad[S.nnz] - 1;
+  oversample_geom(npatches, norders, ixyzs, srccoefs, srcvals, S.novers, S.ixyzso, S.srcover, S.wover);
+  S.nptso = S.ixyzso[npatches] - 1;
+  return 0;
+}
+
+// helm_comb_dir_eval / lap_comb_dir_eval: the layer potential at arbitrary targets
+// (on-surface targets give the principal value part), helm_comb_dir.f:440-652
+int eval_host(bool helm, int64_t npatches, const int64_t* norders, const int64_t* ixyzs, const int64_t* iptype,
+              int64_t npts, const double* srccoefs, const double* srcvals, int64_t ndtarg, int64_t ntarg,
+              const double* targs, const int64_t* ipatch_id, const double* uvs_targ, double eps, const double* kp6,
+              const double* sigma, double* pot) {
+  const int no = uniform_order(npatches, norders, ixyzs, iptype);
+  if (no < 0) return no;
+  NearSetup S;
+  const double babs = std::hypot(kp6[4], kp6[5]);
+  int rc = near_setup(npatches, norders, ixyzs, npts, srccoefs, srcvals, ndtarg, ntarg, targs, eps, kp6, babs, S);
+  if (rc) return rc;
+  std::vector<double> wnear((helm ? 2 : 1) * (S.nquad ? S.nquad : 1), 0.0);
+  if (S.nnz > 0) {
+    rc = nearquad_host(helm, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, ndtarg, ntarg, targs,
+                       ipatch_id, uvs_targ, eps, kp6, S.nnz, S.row_ptr.data(), S.col_ind.data(), S.iquad.data(),
+                       S.rfac0, S.nquad, wnear.data());
+    if (rc) return rc;
+  }
+  return lpcomp_host(helm, npatches, norders, ixyzs, iptype, npts, srcvals, kp6, S.nnz, S.row_ptr.data(),
+                     S.col_ind.data(), S.iquad.data(), S.nquad, wnear.data(), S.novers.data(), S.nptso,
+                     S.ixyzso.data(), S.srcover.data(), S.wover.data(), sigma, pot, ndtarg, ntarg, targs);
+}
 
 extern "C" {
 
@@ -587,6 +759,125 @@ void lap_comb_dir_solver_guru_(const int64_t* npatches, const int64_t* norders, 
   solver_guru_host(false, *npatches, norders, ixyzs, iptype, *npts, srcvals, kp, *numit, *ifinout, rhs, *nnz,
                    row_ptr, col_ind, iquad, *nquad, wnear, novers, *nptso, ixyzso, srcover, whtsover, *eps_gmres,
                    niter, errs, rres, soln);
+}
+
+// ---------------------------------------------------------------- off-surface evaluation
+// helm_comb_dir_eval_addsub (helm_comb_dir.f:856-1208) / lap_comb_dir_eval_addsub
+// (lap_comb_dir.f90:806-1188): the apply at arbitrary targets with precomputed
+// near-quadrature weights; idensflag = ipotflag = 0 and ndim = ndim_p = 1 only.
+static bool eval_flags_ok(const int64_t* nker, const int64_t* idensflag, const int64_t* ndim, const int64_t* ipotflag,
+                          const int64_t* ndim_p) {
+  if (*nker != 1 || *idensflag != 0 || *ndim != 1 || *ipotflag != 0 || *ndim_p != 1) {
+    fprintf(stderr, "fmm3dbie_amd: eval_addsub supports nker = ndim = ndim_p = 1, idensflag = ipotflag = 0\n");
+    g_last_error = -11;
+    return false;
+  }
+  return true;
+}
+
+void helm_comb_dir_eval_addsub_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                                const int64_t* iptype, const int64_t* npts, const double* srccoefs,
+                                const double* srcvals, const int64_t* ndtarg, const int64_t* ntarg,
+                                const double* targs, const double* eps, const int64_t* ndd, const double* dpars,
+                                const int64_t* ndz, const double* zpars, const int64_t* ndi, const int64_t* ipars,
+                                const int64_t* nnz, const int64_t* row_ptr, const int64_t* col_ind,
+                                const int64_t* iquad, const int64_t* nquad, const int64_t* nker, const double* wnear,
+                                const int64_t* novers, const int64_t* nptso, const int64_t* ixyzso,
+                                const double* srcover, const double* whtsover, const int64_t* lwork,
+                                const double* work, const int64_t* idensflag, const int64_t* ndim,
+                                const double* sigma, const int64_t* ipotflag, const int64_t* ndim_p, double* pot) {
+  (void)srccoefs; (void)eps; (void)ndd; (void)dpars; (void)ndz; (void)ndi; (void)ipars; (void)lwork; (void)work;
+  if (!eval_flags_ok(nker, idensflag, ndim, ipotflag, ndim_p)) return;
+  lpcomp_host(true, *npatches, norders, ixyzs, iptype, *npts, srcvals, zpars, *nnz, row_ptr, col_ind, iquad, *nquad,
+              wnear, novers, *nptso, ixyzso, srcover, whtsover, sigma, pot, *ndtarg, *ntarg, targs);
+}
+
+void lap_comb_dir_eval_addsub_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                               const int64_t* iptype, const int64_t* npts, const double* srccoefs,
+                               const double* srcvals, const int64_t* ndtarg, const int64_t* ntarg,
+                               const double* targs, const double* eps, const int64_t* ndd, const double* dpars,
+                               const int64_t* ndz, const double* zpars, const int64_t* ndi, const int64_t* ipars,
+                               const int64_t* nnz, const int64_t* row_ptr, const int64_t* col_ind,
+                               const int64_t* iquad, const int64_t* nquad, const int64_t* nker, const double* wnear,
+                               const int64_t* novers, const int64_t* nptso, const int64_t* ixyzso,
+                               const double* srcover, const double* whtsover, const int64_t* lwork,
+                               const double* work, const int64_t* idensflag, const int64_t* ndim_s,
+                               const double* sigma, const int64_t* ipotflag, const int64_t* ndim_p, double* pot) {
+  (void)srccoefs; (void)eps; (void)ndd; (void)ndz; (void)zpars; (void)ndi; (void)ipars; (void)lwork; (void)work;
+  if (!eval_flags_ok(nker, idensflag, ndim_s, ipotflag, ndim_p)) return;
+  const double kp[6] = {0.0, 0.0, dpars[0], 0.0, dpars[1], 0.0};
+  lpcomp_host(false, *npatches, norders, ixyzs, iptype, *npts, srcvals, kp, *nnz, row_ptr, col_ind, iquad, *nquad,
+              wnear, novers, *nptso, ixyzso, srcover, whtsover, sigma, pot, *ndtarg, *ntarg, targs);
+}
+
+// helm_comb_dir_eval (helm_comb_dir.f:440-652) / lap_comb_dir_eval (lap_comb_dir.f90:408-632):
+// near list, far orders, near quadrature and apply at the targets, all on the GPU
+void helm_comb_dir_eval_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs, const int64_t* iptype,
+                         const int64_t* npts, const double* srccoefs, const double* srcvals, const int64_t* ndtarg,
+                         const int64_t* ntarg, const double* targs, const int64_t* ipatch_id, const double* uvs_targ,
+                         const double* eps, const double* zpars, const double* sigma, double* pot) {
+  g_last_error = eval_host(true, *npatches, norders, ixyzs, iptype, *npts, srccoefs, srcvals, *ndtarg, *ntarg, targs,
+                           ipatch_id, uvs_targ, *eps, zpars, sigma, pot);
+}
+
+void lap_comb_dir_eval_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs, const int64_t* iptype,
+                        const int64_t* npts, const double* srccoefs, const double* srcvals, const int64_t* ndtarg,
+                        const int64_t* ntarg, const double* targs, const int64_t* ipatch_id, const double* uvs_targ,
+                        const double* eps, const double* dpars, const double* sigma, double* pot) {
+  const double kp[6] = {0.0, 0.0, dpars[0], 0.0, dpars[1], 0.0};
+  g_last_error = eval_host(false, *npatches, norders, ixyzs, iptype, *npts, srccoefs, srcvals, *ndtarg, *ntarg,
+                           targs, ipatch_id, uvs_targ, *eps, kp, sigma, pot);
+}
+
+// helm_comb_dir_solver (helm_comb_dir.f:1808-2064) / lap_comb_dir_solver
+// (lap_comb_dir.f90:1189-1458): on-surface setup (near list, far orders, near quadrature)
+// on the GPU, then the device GMRES of the _guru routine
+static void solver_host(bool helm, const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                        const int64_t* iptype, const int64_t* npts, const double* srccoefs, const double* srcvals,
+                        const double* eps, const double* kp6, const int64_t* numit, const int64_t* ifinout,
+                        const double* rhs, const double* eps_gmres, int64_t* niter, double* errs, double* rres,
+                        double* soln) {
+  const int no = uniform_order(*npatches, norders, ixyzs, iptype);
+  if (no < 0) {
+    g_last_error = no;
+    return;
+  }
+  NearSetup S;
+  int rc = near_setup(*npatches, norders, ixyzs, *npts, srccoefs, srcvals, 12, *npts, srcvals, *eps, kp6,
+                      std::hypot(kp6[4], kp6[5]), S);
+  std::vector<int64_t> pid;
+  std::vector<double> uvs;
+  if (!rc) patch_id_uvs(*npatches, norders, ixyzs, pid, uvs);
+  std::vector<double> wnear((helm ? 2 : 1) * (S.nquad ? S.nquad : 1), 0.0);
+  if (!rc && S.nnz > 0)
+    rc = nearquad_host(helm, *npatches, norders, ixyzs, iptype, *npts, srccoefs, srcvals, 12, *npts, srcvals,
+                       pid.data(), uvs.data(), *eps, kp6, S.nnz, S.row_ptr.data(), S.col_ind.data(), S.iquad.data(),
+                       S.rfac0, S.nquad, wnear.data());
+  if (rc) {
+    g_last_error = rc;
+    return;
+  }
+  solver_guru_host(helm, *npatches, norders, ixyzs, iptype, *npts, srcvals, kp6, *numit, *ifinout, rhs, S.nnz,
+                   S.row_ptr.data(), S.col_ind.data(), S.iquad.data(), S.nquad, wnear.data(), S.novers.data(),
+                   S.nptso, S.ixyzso.data(), S.srcover.data(), S.wover.data(), *eps_gmres, niter, errs, rres, soln);
+}
+
+void helm_comb_dir_solver_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs,
+                           const int64_t* iptype, const int64_t* npts, const double* srccoefs, const double* srcvals,
+                           const double* eps, const double* zpars, const int64_t* numit, const int64_t* ifinout,
+                           const double* rhs, const double* eps_gmres, int64_t* niter, double* errs, double* rres,
+                           double* soln) {
+  solver_host(true, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, zpars, numit, ifinout, rhs,
+              eps_gmres, niter, errs, rres, soln);
+}
+
+void lap_comb_dir_solver_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs, const int64_t* iptype,
+                          const int64_t* npts, const double* srccoefs, const double* srcvals, const double* eps,
+                          const double* dpars, const int64_t* numit, const int64_t* ifinout, const double* rhs,
+                          const double* eps_gmres, int64_t* niter, double* errs, double* rres, double* soln) {
+  const double kp[6] = {0.0, 0.0, dpars[0], 0.0, dpars[1], 0.0};
+  solver_host(false, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, kp, numit, ifinout, rhs,
+              eps_gmres, niter, errs, rres, soln);
 }
 
 // ---------------------------------------------------------------- near-field finder

@@ -302,6 +302,10 @@ int gmres_impl(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, voi
 int gmres_device(ApplyPlan* P, int prec, int mode, const double* zid, const void* rhs, void* soln,
                  const double* wts, int nrhs, int numit, double eps_gmres, int* niter, double* errs, double* rres,
                  hipStream_t s, GmresStats* gs) {
+  if (P->ntarg != P->npts) {
+    fprintf(stderr, "fmm3dbie_amd: GMRES needs the on-surface operator (plan targets = discretization points)\n");
+    return -6;
+  }
   if (prec == 64)
     return gmres_impl<double>(P, mode, zid, rhs, soln, wts, nrhs, numit, eps_gmres, niter, errs, rres, s, gs);
   return gmres_impl<float>(P, mode, zid, rhs, soln, wts, nrhs, numit, eps_gmres, niter, errs, rres, s, gs);
@@ -312,6 +316,10 @@ int gmres_device(ApplyPlan* P, int prec, int mode, const double* zid, const void
 int gmres_ir_device(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, void* soln_v,
                     const double* wts, int nrhs, int numit, double eps_inner, double tol, int max_outer,
                     int* outer_out, int* inner_total, double* rres, hipStream_t s, GmresStats* gs) {
+  if (P->ntarg != P->npts) {
+    fprintf(stderr, "fmm3dbie_amd: GMRES needs the on-surface operator (plan targets = discretization points)\n");
+    return -6;
+  }
   const int64_t N = P->npts;
   const C2<double>* rhs = (const C2<double>*)rhs_v;
   C2<double>* x = (C2<double>*)soln_v;

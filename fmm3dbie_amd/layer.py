@@ -9,7 +9,11 @@ libfmm3dbie_amd.so, which run on the GPU:
   (src/lap_wrappers/lap_comb_dir.f90:86-406);
 * :func:`lpcomp_helm_comb_dir_addsub`, :func:`lpcomp_lap_comb_dir_addsub`
   (helm_comb_dir.f:658-850, lap_comb_dir.f90:633-805);
-* :func:`helm_comb_dir_solver_guru`, :func:`lap_comb_dir_solver_guru`.
+* :func:`helm_comb_dir_solver_guru`, :func:`lap_comb_dir_solver_guru`;
+* :func:`helm_comb_dir_solver`, :func:`lap_comb_dir_solver` (setup included,
+  helm_comb_dir.f:1808-2064, lap_comb_dir.f90:1189-1458);
+* :func:`helm_comb_dir_eval`, :func:`lap_comb_dir_eval` (potential at arbitrary targets,
+  helm_comb_dir.f:440-652, lap_comb_dir.f90:408-632).
 
 Device paths (torch tensors resident in HBM) are :class:`NearQuadDevice`,
 :class:`ApplyPlan`, :func:`gmres` and :func:`gmres_ir`.
@@ -185,6 +189,64 @@ def lap_comb_dir_solver_guru(npatches, norders, ixyzs, iptype, npts, srccoefs, s
     return _solver("lap_comb_dir_solver_guru_", False, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals,
                    eps, dpars, numit, ifinout, rhs, nnz, row_ptr, col_ind, iquad, nquad, wnear, novers, nptso,
                    ixyzso, srcover, wover, eps_gmres)
+
+
+def _eval(sym, cplx, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, ndtarg, ntarg, targs, ipatch_id,
+          uvs_targ, eps, pars, sigma):
+    pot = np.zeros(int(ntarg), np.complex128 if cplx else np.float64)
+    keep = [_arr(norders, np.int64), _arr(ixyzs, np.int64), _arr(iptype, np.int64), _arr(srccoefs, np.float64),
+            _arr(srcvals, np.float64), _arr(targs, np.float64), _arr(ipatch_id, np.int64),
+            _arr(uvs_targ, np.float64), _z3(pars) if cplx else _arr(pars, np.float64),
+            _arr(sigma, np.complex128 if cplx else np.float64)]
+    getattr(lib(), sym)(_i8(npatches), _p(keep[0]), _p(keep[1]), _p(keep[2]), _i8(npts), _p(keep[3]), _p(keep[4]),
+                        _i8(ndtarg), _i8(ntarg), _p(keep[5]), _p(keep[6]), _p(keep[7]), _d(eps), _p(keep[8]),
+                        _p(keep[9]), _p(pot))
+    _check(sym)
+    return pot
+
+
+def helm_comb_dir_eval(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, ndtarg, ntarg, targs, ipatch_id,
+                       uvs_targ, eps, zpars, sigma):
+    """(alpha S_k + beta D_k)[sigma] at the targets (helm_comb_dir.f:440-652); off-surface
+    targets have ipatch_id = -1."""
+    return _eval("helm_comb_dir_eval_", True, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, ndtarg,
+                 ntarg, targs, ipatch_id, uvs_targ, eps, zpars, sigma)
+
+
+def lap_comb_dir_eval(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, ndtarg, ntarg, targs, ipatch_id,
+                      uvs_targ, eps, dpars, sigma):
+    return _eval("lap_comb_dir_eval_", False, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, ndtarg,
+                 ntarg, targs, ipatch_id, uvs_targ, eps, dpars, sigma)
+
+
+def _solver_full(sym, cplx, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, pars, numit, ifinout,
+                 rhs, eps_gmres):
+    soln = np.zeros(int(npts), np.complex128 if cplx else np.float64)
+    errs = np.zeros(int(numit) + 1)
+    niter = C.c_int64(0)
+    rres = C.c_double(0)
+    keep = [_arr(norders, np.int64), _arr(ixyzs, np.int64), _arr(iptype, np.int64), _arr(srccoefs, np.float64),
+            _arr(srcvals, np.float64), _z3(pars) if cplx else _arr(pars, np.float64),
+            _arr(rhs, np.complex128 if cplx else np.float64)]
+    getattr(lib(), sym)(_i8(npatches), _p(keep[0]), _p(keep[1]), _p(keep[2]), _i8(npts), _p(keep[3]), _p(keep[4]),
+                        _d(eps), _p(keep[5]), _i8(numit), _i8(ifinout), _p(keep[6]), _d(eps_gmres), C.byref(niter),
+                        _p(errs), C.byref(rres), _p(soln))
+    _check(sym)
+    return int(niter.value), errs[:niter.value], float(rres.value), soln
+
+
+def helm_comb_dir_solver(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, zpars, numit, ifinout, rhs,
+                         eps_gmres):
+    """(niter, errs, rres, soln): setup (near list, far orders, quadrature) and GMRES on the
+    GPU (helm_comb_dir.f:1808-2064)."""
+    return _solver_full("helm_comb_dir_solver_", True, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps,
+                        zpars, numit, ifinout, rhs, eps_gmres)
+
+
+def lap_comb_dir_solver(npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps, dpars, numit, ifinout, rhs,
+                        eps_gmres):
+    return _solver_full("lap_comb_dir_solver_", False, npatches, norders, ixyzs, iptype, npts, srccoefs, srcvals, eps,
+                        dpars, numit, ifinout, rhs, eps_gmres)
 
 
 # ------------------------------------------------------------------ device paths

@@ -124,6 +124,56 @@ void get_iquad_rsc_(const int64_t *npatches, const int64_t *ixyzs, const int64_t
 /* src/quadratures/near_field_routs.f:998-1057  get_near_corr_max */
 void get_near_corr_max_(const int64_t *ntarg, const int64_t *row_ptr, const int64_t *nnz,
                         const int64_t *col_ind, const int64_t *npatches, const int64_t *ixyzs, int64_t *nmax);
+/* src/helm_wrappers/helm_comb_dir.f:856-1208  helm_comb_dir_eval_addsub: the apply at
+ * arbitrary targets (ndtarg, ntarg) with precomputed near weights; nker = ndim = ndim_p = 1,
+ * idensflag = ipotflag = 0 */
+void helm_comb_dir_eval_addsub_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs,
+                                const int64_t *iptype, const int64_t *npts, const double *srccoefs,
+                                const double *srcvals, const int64_t *ndtarg, const int64_t *ntarg,
+                                const double *targs, const double *eps, const int64_t *ndd, const double *dpars,
+                                const int64_t *ndz, const double *zpars, const int64_t *ndi, const int64_t *ipars,
+                                const int64_t *nnz, const int64_t *row_ptr, const int64_t *col_ind,
+                                const int64_t *iquad, const int64_t *nquad, const int64_t *nker, const double *wnear,
+                                const int64_t *novers, const int64_t *nptso, const int64_t *ixyzso,
+                                const double *srcover, const double *whtsover, const int64_t *lwork,
+                                const double *work, const int64_t *idensflag, const int64_t *ndim,
+                                const double *sigma, const int64_t *ipotflag, const int64_t *ndim_p, double *pot);
+/* src/lap_wrappers/lap_comb_dir.f90:806-1188  lap_comb_dir_eval_addsub */
+void lap_comb_dir_eval_addsub_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs,
+                               const int64_t *iptype, const int64_t *npts, const double *srccoefs,
+                               const double *srcvals, const int64_t *ndtarg, const int64_t *ntarg,
+                               const double *targs, const double *eps, const int64_t *ndd, const double *dpars,
+                               const int64_t *ndz, const double *zpars, const int64_t *ndi, const int64_t *ipars,
+                               const int64_t *nnz, const int64_t *row_ptr, const int64_t *col_ind,
+                               const int64_t *iquad, const int64_t *nquad, const int64_t *nker, const double *wnear,
+                               const int64_t *novers, const int64_t *nptso, const int64_t *ixyzso,
+                               const double *srcover, const double *whtsover, const int64_t *lwork,
+                               const double *work, const int64_t *idensflag, const int64_t *ndim_s,
+                               const double *sigma, const int64_t *ipotflag, const int64_t *ndim_p, double *pot);
+/* src/helm_wrappers/helm_comb_dir.f:440-652  helm_comb_dir_eval: layer potential at
+ * targets (near list, far orders, quadrature and apply on the GPU); on-surface targets
+ * give the principal value part */
+void helm_comb_dir_eval_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs, const int64_t *iptype,
+                         const int64_t *npts, const double *srccoefs, const double *srcvals, const int64_t *ndtarg,
+                         const int64_t *ntarg, const double *targs, const int64_t *ipatch_id, const double *uvs_targ,
+                         const double *eps, const double *zpars, const double *sigma, double *pot);
+/* src/lap_wrappers/lap_comb_dir.f90:408-632  lap_comb_dir_eval */
+void lap_comb_dir_eval_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs, const int64_t *iptype,
+                        const int64_t *npts, const double *srccoefs, const double *srcvals, const int64_t *ndtarg,
+                        const int64_t *ntarg, const double *targs, const int64_t *ipatch_id, const double *uvs_targ,
+                        const double *eps, const double *dpars, const double *sigma, double *pot);
+/* src/helm_wrappers/helm_comb_dir.f:1808-2064  helm_comb_dir_solver: setup on the GPU,
+ * then the device GMRES of helm_comb_dir_solver_guru */
+void helm_comb_dir_solver_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs,
+                           const int64_t *iptype, const int64_t *npts, const double *srccoefs, const double *srcvals,
+                           const double *eps, const double *zpars, const int64_t *numit, const int64_t *ifinout,
+                           const double *rhs, const double *eps_gmres, int64_t *niter, double *errs, double *rres,
+                           double *soln);
+/* src/lap_wrappers/lap_comb_dir.f90:1189-1458  lap_comb_dir_solver */
+void lap_comb_dir_solver_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs, const int64_t *iptype,
+                          const int64_t *npts, const double *srccoefs, const double *srcvals, const double *eps,
+                          const double *dpars, const int64_t *numit, const int64_t *ifinout, const double *rhs,
+                          const double *eps_gmres, int64_t *niter, double *errs, double *rres, double *soln);
 /* src/quadratures/near_field_routs.f:193-423  findnearmem: nnz of the near list
  * (sources j with |x_i - c_j|^2 <= rads_j^2), computed on the GPU */
 void findnearmem_(const double *xyzs, const int64_t *ns, const double *rads, const int64_t *ndt,
