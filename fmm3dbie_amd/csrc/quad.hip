@@ -255,63 +255,75 @@ __global__ void __launch_bounds__(128) k_fixed(int nf, const double* __restrict_
   }
 }
 
-// Moments of a set of staged nodes: row q of buf holds (f_re, f_im, P_0..P_{NP-1}) of
-// node q (written by the lane that evaluated it); lane c (< 2NP for complex) sums column c.
-// Single-group variant (lane c < NC sums column c over all rows): fewer registers, used
-// by the self kernel.
-template <int NP, bool HELM>
-__device__ __forceinline__ double column_sum_serial(const double* buf, int nrows) {
-  constexpr int W = NP + 2;
-  constexpr int NC = HELM ? 2 * NP : NP;
-  const int lane = threadIdx.x;
-  __syncthreads();
-  double s = 0.0;
-  if (lane < NC) {
-    const int j = lane % NP, part = lane / NP;
-#pragma unroll 4
-    for (int q = 0; q < nrows; q++) s = fma(buf[q * W + part], buf[q * W + 2 + j], s);
-  }
-  __syncthreads();
-  return s;
-}
+// LDS staging row of one node: f_re, f_im, P_0..P_{NP-1}, padded to an even number of
+// doubles so that every row starts on a 16-byte boundary and (f_re, f_im) is one 128-bit read.
+__host__ __device__ constexpr int row_w(int np) { return (np + 3) & ~1; }
 
-// With NC = 2NP (complex) or NP columns, G = 64 / NC lane groups split the rows (group g
-// takes rows g, g + G, ...) and lane c adds the partial sums of lanes c + k NC.
+// One complex moment component per lane: lane j < NP holds M_j (im = 0 for Laplace).
+struct Mom {
+  double re, im;
+  __device__ Mom& operator+=(const Mom& o) {
+    re += o.re;
+    im += o.im;
+    return *this;
+  }
+};
+
+// Moments of a set of staged nodes, M_j = sum_q f_q P_j(q): row q of buf holds
+// (f_re, f_im, P_0..P_{NP-1}) of node q (written by the lane that evaluated it).  The
+// G = 64 / NP lane groups split the rows (lane g NP + j takes column j of rows g, g + G,
+// ...), one basis read serving the real and imaginary parts; lane j < NP adds the other
+// groups' partial sums and returns M_j.
 template <int NP, bool HELM>
-__device__ __forceinline__ double column_sum(const double* buf, int nrows) {
-  constexpr int W = NP + 2;
-  constexpr int NC = HELM ? 2 * NP : NP;
-  constexpr int G = 64 / NC;
+__device__ __forceinline__ Mom column_sum(const double* buf, int nrows) {
+  static_assert(NP <= WAVE, "one moment component per lane");
+  constexpr int W = row_w(NP);
+  constexpr int G = WAVE / NP;
   const int lane = threadIdx.x;
   __syncthreads();
-  double s = 0.0;
-  const int grp = lane / NC, c = lane - grp * NC;
+  const int grp = lane / NP, j = lane - grp * NP;
+  double sr = 0.0, si = 0.0;
   if (grp < G) {
-    const int j = c % NP, part = c / NP;
 #pragma unroll 4
-    for (int q = grp; q < nrows; q += G) s = fma(buf[q * W + part], buf[q * W + 2 + j], s);
+    for (int q = grp; q < nrows; q += G) {
+      const double* row = buf + q * W;
+      const double p = row[2 + j];
+      if (HELM) {
+        const double2 f = *reinterpret_cast<const double2*>(row);
+        sr = fma(f.x, p, sr);
+        si = fma(f.y, p, si);
+      } else {
+        sr = fma(row[0], p, sr);
+      }
+    }
   }
-  const double part_sum = s;  // shuffle the unmodified partial sums
+  const double pr = sr, pi = si;  // shuffle the unmodified partial sums
 #pragma unroll
-  for (int k = 1; k < G; k++) s += __shfl(part_sum, (lane + k * NC) & 63, 64);
+  for (int k = 1; k < G; k++) {
+    sr += __shfl(pr, (lane + k * NP) & 63, 64);
+    if (HELM) si += __shfl(pi, (lane + k * NP) & 63, 64);
+  }
   __syncthreads();
-  return lane < NC ? s : 0.0;
+  return lane < NP ? Mom{sr, HELM ? si : 0.0} : Mom{0.0, 0.0};
 }
 
-// Apply umat^T to per-lane moment components and store the weights of one pair.
+// Apply umat^T to the moments (lane j < NP holds M_j) and store the weights of one pair;
+// output c is weight c/2, part c%2 (complex) or weight c (real), so the stores coalesce.
 template <int NP, bool HELM>
-__device__ __forceinline__ void store_weights(double* buf, double mom, const double* __restrict__ umat,
+__device__ __forceinline__ void store_weights(double* buf, Mom mom, const double* __restrict__ umat,
                                               double* __restrict__ wnear, int64_t q0) {
   const int lane = threadIdx.x;
   constexpr int NC = HELM ? 2 * NP : NP;
-  if (lane < NC) buf[lane] = mom;
+  if (lane < NP) {
+    buf[lane] = mom.re;
+    if (HELM) buf[NP + lane] = mom.im;
+  }
   __syncthreads();
-  if (lane < NC) {
-    const int j = lane % NP, part = lane / NP;
+  for (int c = lane; c < NC; c += WAVE) {
+    const int j = HELM ? c >> 1 : c, part = HELM ? c & 1 : 0;
     double s = 0.0;
     for (int l = 0; l < NP; l++) s = fma(umat[l + j * NP], buf[part * NP + l], s);
-    if (HELM) wnear[2 * (q0 + j) + part] = s;
-    else wnear[q0 + j] = s;
+    wnear[(HELM ? 2 : 1) * q0 + c] = s;
   }
   __syncthreads();
 }
@@ -319,13 +331,13 @@ __device__ __forceinline__ void store_weights(double* buf, double mom, const dou
 // ------------------------------------------------------------------ 4. adaptive
 // Moments over one tree triangle (x0,y0) + diag(dx,dy)*(u,v) of the XG(kq) rule.
 template <int N, bool HELM>
-__device__ __forceinline__ double tri_moments(double* buf, const double* sc, double x0, double y0, double dx,
-                                              double dy, double da, int kq, const double* __restrict__ xg,
-                                              const double* tg, const KPar& kp) {
+__device__ __forceinline__ Mom tri_moments(double* buf, const double* sc, double x0, double y0, double dx,
+                                           double dy, double da, int kq, const double* __restrict__ xg,
+                                           const double* tg, const KPar& kp) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
-  constexpr int W = NP + 2;
+  constexpr int W = row_w(NP);
   const int lane = threadIdx.x;
-  double total = 0.0;
+  Mom total{0.0, 0.0};
   for (int base = 0; base < kq; base += WAVE) {
     const int q = base + lane;
     if (q < kq) {
@@ -393,18 +405,17 @@ __global__ void k_tree_cache(int64_t npatches, int kq, const double* __restrict_
 // stages (f_q, P_j(q)) in LDS, the basis values copied from the shared table; lane c
 // then sums column c
 template <int N, bool HELM>
-__device__ __forceinline__ double tri_moments_cached(double* buf, const double* __restrict__ cpatch, int nn,
-                                                     const double* __restrict__ ptab, int tri, int kq,
-                                                     const double* tg, const KPar& kp) {
+__device__ __forceinline__ Mom tri_moments_cached(double* buf, const double* __restrict__ cpatch, int nn,
+                                                  const double* __restrict__ ptab, int tri, int kq,
+                                                  const double* tg, const KPar& kp) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
-  constexpr int NC = HELM ? 2 * NP : NP;
   const int lane = threadIdx.x;
-  double total = 0.0;
+  Mom total{0.0, 0.0};
   for (int base = 0; base < kq; base += WAVE) {
     const int q = base + lane;
     if (q < kq) {
       const int node = tri * kq + q;
-      double* row = buf + lane * (NP + 2);
+      double* row = buf + lane * row_w(NP);
       for (int l = 0; l < NP; l++) row[2 + l] = ptab[(size_t)node * NP + l];
       double g[12];
       const double* cn = cpatch + (size_t)CACHE_STRIDE * node;
@@ -442,11 +453,10 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
                                                  const double* __restrict__ cache, const double* __restrict__ ptab) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
   constexpr int ES = 6 + 2 * NP;  // stack entry: x0,y0,dx,dy,level,pad, moments
-  __shared__ double buf[WAVE * (NP + 2)];
+  __shared__ __align__(16) double buf[WAVE * row_w(NP)];
   __shared__ int sh_idx;
   const int lane = threadIdx.x;
   double* stack = stack_all + (size_t)blockIdx.x * STACK_CAP * ES;
-  constexpr int NC = HELM ? 2 * NP : NP;
   // Work queue: the list (sorted by patch) in 8 contiguous segments, one per XCD
   // (workgroups are dispatched to the XCDs round-robin, so workgroup b starts on segment
   // b % 8 and the patch geometry it reads stays in its XCD's L2), then the other
@@ -485,12 +495,15 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
     const int nn = NCACHE_TRI * kq;
     const double* cpatch = cache + (size_t)ip * CACHE_STRIDE * nn;
     // root triangle (cached tree id 0)
-    double mroot = tri_moments_cached<N, HELM>(buf, cpatch, nn, ptab, 0, kq, tg, kp);
+    const Mom mroot = tri_moments_cached<N, HELM>(buf, cpatch, nn, ptab, 0, kq, tg, kp);
     int sp = 0;
     if (lane < 6) stack[lane] = lane < 2 ? 0.0 : (lane < 4 ? 1.0 : 0.0);  // level 0, tree id 0
-    if (lane < NC) stack[6 + lane] = mroot;
+    if (lane < NP) {
+      stack[6 + lane] = mroot.re;
+      stack[6 + NP + lane] = mroot.im;
+    }
     sp = 1;
-    double result = 0.0;
+    Mom result{0.0, 0.0};
     int nref = 0;
     while (sp > 0) {
       sp--;
@@ -498,19 +511,19 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
       const double x0 = e[0], y0 = e[1], dx = e[2], dy = e[3];
       const int level = (int)e[4];
       const int tid = (int)e[5];  // cached tree id, -1 below depth 2
-      const double mpar = lane < NC ? e[6 + lane] : 0.0;
+      const Mom mpar = lane < NP ? Mom{e[6 + lane], e[6 + NP + lane]} : Mom{0.0, 0.0};
       const double hx = 0.5 * dx, hy = 0.5 * dy, da = hx * hx;  // child area ratio 4^-(level+1)
       const double cx[4] = {x0, x0 + hx, x0, x0 + hx};
       const double cy[4] = {y0, y0, y0 + hy, y0 + hy};
       const double cdx[4] = {hx, hx, hx, -hx};
       const double cdy[4] = {hy, hy, hy, -hy};
-      double ssum = 0.0;
+      Mom ssum{0.0, 0.0};
       // the four children go to stack slots sp..sp+3 (committed below only if refined)
       const bool room = sp + 4 <= STACK_CAP;
 #pragma unroll 1
       for (int c = 0; c < 4; c++) {
         const int ctid = cached_child(tid, c);
-        const double mcc = ctid >= 0
+        const Mom mcc = ctid >= 0
                                ? tri_moments_cached<N, HELM>(buf, cpatch, nn, ptab, ctid, kq, tg, kp)
                                : tri_moments<N, HELM>(buf, sc, cx[c], cy[c], cdx[c], cdy[c], da, kq, xg, tg, kp);
         ssum += mcc;
@@ -520,20 +533,17 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
             ne[0] = cx[c]; ne[1] = cy[c]; ne[2] = cdx[c]; ne[3] = cdy[c]; ne[4] = (double)(level + 1);
             ne[5] = (double)(ctid >= 0 && ctid < NCACHE_PARENT ? ctid : -1);
           }
-          if (lane < NC) ne[6 + lane] = mcc;
+          if (lane < NP) {
+            ne[6 + lane] = mcc.re;
+            ne[6 + NP + lane] = mcc.im;
+          }
         }
       }
       nref++;
       // error = max_j |sum children - parent| (complex modulus)
-      const double dlt = ssum - mpar;
-      double err;
-      if (HELM) {
-        const double other = __shfl(dlt, (lane + NP) & 63, 64);
-        err = lane < NP ? sqrt(dlt * dlt + other * other) : 0.0;
-      } else {
-        err = lane < NP ? fabs(dlt) : 0.0;
-      }
-      err = wave_max(err);
+      const double dre = ssum.re - mpar.re, dim = ssum.im - mpar.im;
+      double err = HELM ? sqrt(dre * dre + dim * dim) : fabs(dre);
+      err = wave_max(lane < NP ? err : 0.0);
       if (err > eps && level + 1 <= NLMAX && room) {
         sp += 4;
       } else {
@@ -910,7 +920,7 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
                                              const double* __restrict__ umat, KPar kp, int nr, int nt,
                                              double tau_panel, double* __restrict__ wnear) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
-  __shared__ double buf[WAVE * (NP + 2)];
+  __shared__ __align__(16) double buf[WAVE * row_w(NP)];
   const int lane = threadIdx.x;
   const int idx = blockIdx.x;
   if (idx >= nself) return;
@@ -929,7 +939,7 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
   const double l11 = sqrt(g11), l12 = g12 / l11, l22 = sqrt(g22 - l12 * l12);
   const double detl = l11 * l22;
   const double vx[3] = {0.0, 1.0, 0.0}, vy[3] = {0.0, 0.0, 1.0};
-  double acc = 0.0;
+  Mom acc{0.0, 0.0};
   const int nn = nr * nt;
   const float inv_nr = 1.0f / (float)nr;
   for (int e = 0; e < 3; e++) {
@@ -958,7 +968,7 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
         for (int base = 0; base < nn; base += WAVE) {
           const int k = base + lane;
           if (k < nn) {
-            double* row = buf + lane * (NP + 2);
+            double* row = buf + lane * row_w(NP);
             // k < 256: a float reciprocal gives the exact quotient
             const int it = (int)(((float)k + 0.5f) * inv_nr), ir = k - it * nr;
             const double tau = t0 + (t1 - t0) * g_self_gl.x[it];
@@ -1350,9 +1360,9 @@ int dispatch(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     case 7: return run_order<7, HELM>(A, st, s);
     case 8: return run_order<8, HELM>(A, st, s);
     case 9: return run_order<9, HELM>(A, st, s);
-    case 10: return run_order<10, HELM>(A, st, s);
     default:
-      fprintf(stderr, "fmm3dbie_amd: patch order %d not supported (1..10)\n", A.norder);
+      // one moment component per lane of a wavefront: (N + 1)(N + 2) / 2 <= 64
+      fprintf(stderr, "fmm3dbie_amd: patch order %d not supported by the near quadrature (1..9)\n", A.norder);
       return -4;
   }
 }

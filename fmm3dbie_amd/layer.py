@@ -268,6 +268,11 @@ def kpar6(helm, pars):
     return np.array([0.0, 0.0, float(a), 0.0, float(b), 0.0])
 
 
+# the adaptive and self kernels hold one complex moment component per lane of a 64-wide
+# wavefront: (p + 1)(p + 2) / 2 <= 64
+MAX_QUAD_ORDER = 9
+
+
 class NearQuadDevice:
     """Near quadrature with device-resident inputs (the benchmarked path).
 
@@ -285,6 +290,9 @@ class NearQuadDevice:
         if not np.all(norders == norders[0]):
             raise ValueError("patches of one order only")
         self.norder = int(norders[0])
+        if not 1 <= self.norder <= MAX_QUAD_ORDER:
+            raise ValueError("patch order %d not supported by the near quadrature (1..%d)"
+                             % (self.norder, MAX_QUAD_ORDER))
         self.norder_branch = near.norder_avg(geom)
         self.npatches = len(norders)
         if targets is None:

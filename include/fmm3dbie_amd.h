@@ -10,7 +10,7 @@
  * return; here a nonzero status of the last call is available from
  * fmm3dbie_last_error().
  *
- * Supported: triangular RV patches (iptype = 1) of one order in 1..10, iquadtype = 1,
+ * Supported: triangular RV patches (iptype = 1) of one order in 1..9, iquadtype = 1,
  * nker = 1, ndim = 1.
  */
 #ifndef FMM3DBIE_AMD_H

@@ -71,3 +71,15 @@ def test_bookkeeping_entry_points():
     L.get_fmm_thresh_(i8(3), i8(2), C.c_void_p(src.ctypes.data), i8(3), i8(1), C.c_void_p(trg.ctypes.data),
                       C.byref(th))
     assert th.value == 2.0 ** -51 * 3.0
+
+
+def test_nearquad_rejects_unsupported_order():
+    """Order 10 has 66 basis functions, more than the lanes of a wavefront: refused on the
+    host before any device work."""
+    import pytest
+    from fmm3dbie_amd import geometry as G, layer
+    g = G.get_sphere_npat(1.0, 1, (0, 0, 0), 10)
+    n = len(g["norders"])
+    with pytest.raises(ValueError, match="order 10"):
+        layer.NearQuadDevice(g, True, np.array([1.1, 1.1j, 1.0]), 1e-6, np.zeros(n + 1, np.int64),
+                             np.zeros(0, np.int64), np.ones(1, np.int64), device="cpu")
