@@ -167,8 +167,10 @@ __device__ __forceinline__ double geom_eval(const double* __restrict__ sc, const
 
 // Basis into an LDS staging row, then geometry from that row:
 //   row[l] = P_l(u,v) (koornf_pols ordering),  g[m] = sum_l sc[9l+m] * row[l].
-// The geometry loop is kept rolled so the 9*NP coefficient reads stay next to their
-// use instead of being hoisted into registers.
+// With the coefficients in global memory (scalar loads) the geometry loop is kept rolled
+// so the 9*NP reads stay next to their use instead of being hoisted into registers; with
+// the coefficients staged in LDS (SC_LDS) it is unrolled by 3, which overlaps the reads
+// of three basis functions.
 template <int N>
 __device__ __forceinline__ void koorn_row(double u, double v, double* row) {
   const double z = 2.0 * u + v - 1.0, y = 1.0 - v, x = 1.0 - 2.0 * v;
@@ -198,24 +200,33 @@ __device__ __forceinline__ void koorn_row(double u, double v, double* row) {
   }
 }
 
-template <int NP>
+template <int NP, bool SC_LDS>
 __device__ __forceinline__ void geom_from_row(const double* __restrict__ sc, const double* __restrict__ row,
                                               double* g) {
 #pragma unroll
   for (int m = 0; m < 9; m++) g[m] = 0.0;
-#pragma unroll 1
-  for (int l = 0; l < NP; l++) {
-    const double p = row[l];
+  if (SC_LDS) {
+#pragma unroll 3
+    for (int l = 0; l < NP; l++) {
+      const double p = row[l];
 #pragma unroll
-    for (int m = 0; m < 9; m++) g[m] = fma(sc[9 * l + m], p, g[m]);
+      for (int m = 0; m < 9; m++) g[m] = fma(sc[9 * l + m], p, g[m]);
+    }
+  } else {
+#pragma unroll 1
+    for (int l = 0; l < NP; l++) {
+      const double p = row[l];
+#pragma unroll
+      for (int m = 0; m < 9; m++) g[m] = fma(sc[9 * l + m], p, g[m]);
+    }
   }
 }
 
-template <int N>
+template <int N, bool SC_LDS = false>
 __device__ __forceinline__ void koorn_geom_row(double u, double v, const double* __restrict__ sc, double* g,
                                                double* row) {
   koorn_row<N>(u, v, row);
-  geom_from_row<(N + 1) * (N + 2) / 2>(sc, row, g);
+  geom_from_row<(N + 1) * (N + 2) / 2, SC_LDS>(sc, row, g);
 }
 
 // normal and area element from g[3..8]; returns |x_u x x_v|, writes unit normal to g[9..11]
@@ -232,6 +243,14 @@ __device__ __forceinline__ double geom_finish(double* g) {
 }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// value of the first active lane (for values known to be equal across the wavefront)
+__device__ __forceinline__ double wave_uniform(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {

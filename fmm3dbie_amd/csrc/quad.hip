@@ -284,7 +284,7 @@ __device__ __forceinline__ Mom column_sum(const double* buf, int nrows) {
   const int grp = lane / NP, j = lane - grp * NP;
   double sr = 0.0, si = 0.0;
   if (grp < G) {
-#pragma unroll 4
+#pragma unroll 8
     for (int q = grp; q < nrows; q += G) {
       const double* row = buf + q * W;
       const double p = row[2 + j];
@@ -344,7 +344,7 @@ __device__ __forceinline__ Mom tri_moments(double* buf, const double* sc, double
       double* row = buf + lane * W;
       const double u = x0 + dx * xg[3 * q], v = y0 + dy * xg[3 * q + 1];
       double g[12];
-      koorn_geom_row<N>(u, v, sc, g, row + 2);
+      koorn_geom_row<N, true>(u, v, sc, g, row + 2);
       const double jac = geom_finish(g);
       const double w = jac * da * xg[3 * q + 2];
       double vr, vi;
@@ -454,6 +454,7 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
   constexpr int NP = (N + 1) * (N + 2) / 2;
   constexpr int ES = 6 + 2 * NP;  // stack entry: x0,y0,dx,dy,level,pad, moments
   __shared__ __align__(16) double buf[WAVE * row_w(NP)];
+  __shared__ double lsc[9 * NP];
   __shared__ int sh_idx;
   const int lane = threadIdx.x;
   double* stack = stack_all + (size_t)blockIdx.x * STACK_CAP * ES;
@@ -488,7 +489,13 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
     if (idx < 0) break;
     const int2 jt = idx < nlist ? list[idx] : fb_list[idx - nlist];
     const int64_t ip = col_ind[jt.x] - 1;
-    const double* sc = srccoefs + 9 * (ixyzs[ip] - 1);
+    // patch coefficients staged in LDS for the geometry of the triangles below the cached
+    // tree (read after the barriers of the root triangle's moments)
+    {
+      const double* gsc = srccoefs + 9 * (ixyzs[ip] - 1);
+      for (int i = lane; i < 9 * NP; i += WAVE) lsc[i] = gsc[i];
+    }
+    const double* sc = lsc;
     const double* tp = targs + ndtarg * (int64_t)jt.y;
     const double tg[3] = {tp[0], tp[1], tp[2]};
     const double eps = epsp_patch[ip];
@@ -508,29 +515,30 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
     while (sp > 0) {
       sp--;
       const double* e = stack + (size_t)sp * ES;
-      const double x0 = e[0], y0 = e[1], dx = e[2], dy = e[3];
-      const int level = (int)e[4];
-      const int tid = (int)e[5];  // cached tree id, -1 below depth 2
+      // the entry's triangle is wave-uniform: kept in scalar registers
+      const double x0 = wave_uniform(e[0]), y0 = wave_uniform(e[1]), dx = wave_uniform(e[2]), dy = wave_uniform(e[3]);
+      const int level = __builtin_amdgcn_readfirstlane((int)e[4]);
+      const int tid = __builtin_amdgcn_readfirstlane((int)e[5]);  // cached tree id, -1 below depth 2
       const Mom mpar = lane < NP ? Mom{e[6 + lane], e[6 + NP + lane]} : Mom{0.0, 0.0};
       const double hx = 0.5 * dx, hy = 0.5 * dy, da = hx * hx;  // child area ratio 4^-(level+1)
-      const double cx[4] = {x0, x0 + hx, x0, x0 + hx};
-      const double cy[4] = {y0, y0, y0 + hy, y0 + hy};
-      const double cdx[4] = {hx, hx, hx, -hx};
-      const double cdy[4] = {hy, hy, hy, -hy};
+
       Mom ssum{0.0, 0.0};
       // the four children go to stack slots sp..sp+3 (committed below only if refined)
       const bool room = sp + 4 <= STACK_CAP;
 #pragma unroll 1
       for (int c = 0; c < 4; c++) {
         const int ctid = cached_child(tid, c);
+        // child c: corner (x0, y0) + ((c & 1) hx, (c >> 1) hy), legs (hx, hy), the fourth reversed
+        const double cxc = (c & 1) ? x0 + hx : x0, cyc = (c & 2) ? y0 + hy : y0;
+        const double cdxc = c == 3 ? -hx : hx, cdyc = c == 3 ? -hy : hy;
         const Mom mcc = ctid >= 0
                                ? tri_moments_cached<N, HELM>(buf, cpatch, nn, ptab, ctid, kq, tg, kp)
-                               : tri_moments<N, HELM>(buf, sc, cx[c], cy[c], cdx[c], cdy[c], da, kq, xg, tg, kp);
+                               : tri_moments<N, HELM>(buf, sc, cxc, cyc, cdxc, cdyc, da, kq, xg, tg, kp);
         ssum += mcc;
         if (room) {
           double* ne = stack + (size_t)(sp + c) * ES;
           if (lane == 0) {
-            ne[0] = cx[c]; ne[1] = cy[c]; ne[2] = cdx[c]; ne[3] = cdy[c]; ne[4] = (double)(level + 1);
+            ne[0] = cxc; ne[1] = cyc; ne[2] = cdxc; ne[3] = cdyc; ne[4] = (double)(level + 1);
             ne[5] = (double)(ctid >= 0 && ctid < NCACHE_PARENT ? ctid : -1);
           }
           if (lane < NP) {
