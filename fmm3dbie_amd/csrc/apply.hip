@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <vector>
 
@@ -65,8 +66,8 @@ __global__ void k_oversample(int64_t nptso, const int* __restrict__ opatch, cons
   }
 }
 
-// e^{ikr}/r and the dipole factor for a real or complex wavenumber; T = double or float
-template <typename T>
+// e^{ikr}/r and the dipole factor; T = double or float; CK: complex wavenumber
+template <typename T, bool CK>
 __device__ __forceinline__ void pair_kernel(T dx, T dy, T dz, T nx, T ny, T nz, T kr, T ki, T& er, T& ei,
                                             T& tdr, T& tdi, T& r2out) {
   const T r2 = dx * dx + dy * dy + dz * dz;
@@ -87,7 +88,7 @@ __device__ __forceinline__ void pair_kernel(T dx, T dy, T dz, T nx, T ny, T nz, 
     sincos(kr * r, &s, &c);
   }
   T mag = rinv;
-  if (ki != (T)0) {
+  if constexpr (CK) {
     if constexpr (sizeof(T) == 4) mag *= __expf(-ki * r);
     else mag *= exp(-ki * r);
   }
@@ -95,26 +96,28 @@ __device__ __forceinline__ void pair_kernel(T dx, T dy, T dz, T nx, T ny, T nz, 
   ei = s * mag;
   // dipole factor t*(1 - i k r), t = d/r^2, k = kr + i ki: (1 + ki r) - i kr r
   const T t = d * rinv * rinv;
-  tdr = t * ((T)1 + ki * r);
+  tdr = CK ? t * ((T)1 + ki * r) : t;
   tdi = -t * kr * r;
 }
 
 // Tiled far sum.  pot[i] (+)= sum_j [q_j + m_j * t_ij (1-ikr)] e^{ikr}/r / (4 pi)
-// MASK: skip sources whose patch is in the target's sorted near list.
-template <typename T, bool MASK, int NRHS>
+// MASK: skip sources whose patch is in the target's sorted near list.  Sources are sorted
+// by patch, so each thread keeps the source interval [ea, eb) of the next near patch
+// (patch p owns sources ixyzso[p]-1 .. ixyzso[p+1]-2) and moves it forward when the tile
+// passes eb; the pair loop itself has no branches (a skipped pair adds an exact zero).
+template <typename T, bool MASK, bool CK, int NRHS>
 __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restrict__ tx, const T* __restrict__ ty,
                                                 const T* __restrict__ tz, int64_t nsrc,
                                                 const T* __restrict__ sx, const T* __restrict__ sy,
                                                 const T* __restrict__ sz, const T* __restrict__ snx,
                                                 const T* __restrict__ sny, const T* __restrict__ snz,
-                                                const int* __restrict__ spatch, const V2<T>* __restrict__ qstr,
+                                                const int64_t* __restrict__ ixyzso, const V2<T>* __restrict__ qstr,
                                                 const V2<T>* __restrict__ mstr, int64_t ldstr,
                                                 const int64_t* __restrict__ row_ptr,
                                                 const int* __restrict__ near_sorted, T kr, T ki, T thresh2,
                                                 int64_t chunk, V2<T>* __restrict__ part, int64_t ldpart) {
   __shared__ T lx[TPB], ly[TPB], lz[TPB], lnx[TPB], lny[TPB], lnz[TPB];
   __shared__ V2<T> lq[NRHS][TPB], lm[NRHS][TPB];
-  __shared__ int lp[TPB];
   // blockIdx.z selects a group of NRHS right-hand sides: the geometry and e^{ikr} of a
   // pair are computed once per group
   const int64_t rg = blockIdx.z;
@@ -128,14 +131,27 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
   for (int r = 0; r < NRHS; r++) { accr[r] = 0; acci[r] = 0; }
   const int64_t jbeg = blockIdx.y * chunk;
   const int64_t jend = jbeg + chunk < nsrc ? jbeg + chunk : nsrc;
-  int64_t nptr = 0, nend = 0;
-  int nextnear = 0x7fffffff;
+  constexpr int64_t NONE = INT64_MAX;
+  int64_t nptr = 0, nend = 0, ea = NONE, eb = NONE;
+  // first near patch of the target whose sources end after j
+  auto advance = [&](int64_t j) {
+    ea = NONE;
+    eb = NONE;
+    while (nptr < nend) {
+      const int p = near_sorted[nptr];
+      const int64_t e = ixyzso[p + 1] - 1;
+      if (e > j) {
+        ea = ixyzso[p] - 1;
+        eb = e;
+        break;
+      }
+      nptr++;
+    }
+  };
   if (MASK && act && jbeg < jend) {
     nptr = row_ptr[i] - 1;
     nend = row_ptr[i + 1] - 1;
-    const int p0 = spatch[jbeg];
-    while (nptr < nend && near_sorted[nptr] < p0) nptr++;
-    nextnear = nptr < nend ? near_sorted[nptr] : 0x7fffffff;
+    advance(jbeg);
   }
   for (int64_t base = jbeg; base < jend; base += TPB) {
     const int64_t j = base + threadIdx.x;
@@ -147,30 +163,32 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
         lq[r][threadIdx.x] = qstr[r * ldstr + j];
         lm[r][threadIdx.x] = mstr[r * ldstr + j];
       }
-      if (MASK) lp[threadIdx.x] = spatch[j];
     }
     __syncthreads();
     const int cnt = jend - base < TPB ? (int)(jend - base) : TPB;
+#pragma unroll 2
     for (int k = 0; k < cnt; k++) {
-      if (MASK) {
-        const int pk = lp[k];
-        while (nextnear < pk) {
-          nptr++;
-          nextnear = nptr < nend ? near_sorted[nptr] : 0x7fffffff;
-        }
-        if (nextnear == pk) continue;
-      }
+      const int64_t jk = base + k;
+      if (MASK && jk >= eb) advance(jk);
       T er, ei, tdr, tdi, r2;
-      pair_kernel<T>(x - lx[k], y - ly[k], z - lz[k], lnx[k], lny[k], lnz[k], kr, ki, er, ei, tdr, tdi, r2);
-      if (r2 <= thresh2) continue;
+      pair_kernel<T, CK>(x - lx[k], y - ly[k], z - lz[k], lnx[k], lny[k], lnz[k], kr, ki, er, ei, tdr, tdi, r2);
+      const bool keep = (!MASK || jk < ea) && r2 > thresh2;
+      constexpr bool SEL_FACTORS = NRHS > 2;  // zero the pair's factors once, not each sum
+      if (SEL_FACTORS) {
+        er = keep ? er : (T)0;
+        ei = keep ? ei : (T)0;
+        tdr = keep ? tdr : (T)0;
+        tdi = keep ? tdi : (T)0;
+      }
 #pragma unroll
       for (int r = 0; r < NRHS; r++) {
         const V2<T> q = lq[r][k], m = lm[r][k];
         // inner = q + m*(tdr + i tdi)
         const T ir = q.x + m.x * tdr - m.y * tdi;
         const T ii = q.y + m.x * tdi + m.y * tdr;
-        accr[r] += ir * er - ii * ei;
-        acci[r] += ir * ei + ii * er;
+        const T cr = ir * er - ii * ei, ci = ir * ei + ii * er;
+        accr[r] += SEL_FACTORS || keep ? cr : (T)0;
+        acci[r] += SEL_FACTORS || keep ? ci : (T)0;
       }
     }
     __syncthreads();
@@ -218,7 +236,7 @@ __global__ void k_near_sub(int64_t ntarg, const double* __restrict__ tx, const d
       const int64_t ip = col_ind[jj] - 1;
       for (int64_t j = ixyzso[ip] - 1 + lane; j < ixyzso[ip + 1] - 1; j += 64) {
         double er, ei, tdr, tdi, r2;
-        pair_kernel<double>(x - sx[j], y - sy[j], z - sz[j], snx[j], sny[j], snz[j], kr, ki, er, ei, tdr, tdi, r2);
+        pair_kernel<double, true>(x - sx[j], y - sy[j], z - sz[j], snx[j], sny[j], snz[j], kr, ki, er, ei, tdr, tdi, r2);
         if (r2 <= thresh2) continue;
         const V2<double> q = qstr[r * ldstr + j], m = mstr[r * ldstr + j];
         const double ir = q.x + m.x * tdr - m.y * tdi;
@@ -597,14 +615,19 @@ static int launch_far(ApplyPlan* P, bool mask, int64_t wg_target, const V2<T>* q
   nsplit = (P->nptso + chunk - 1) / chunk;
   const int64_t ldpart = (int64_t)nrhs * P->ntarg;
   dim3 grid(g, (unsigned)nsplit, (unsigned)ngrp);
-  if (mask)
-    k_farsum<T, true, NRHS><<<grid, TPB, 0, s>>>(P->ntarg, trg[0], trg[1], trg[2], P->nptso, src[0], src[1], src[2],
-                                                 src[3], src[4], src[5], P->opatch, q, m, ldstr, P->row_ptr,
-                                                 P->near_sorted, kr, ki, th2, chunk, part, ldpart);
-  else
-    k_farsum<T, false, NRHS><<<grid, TPB, 0, s>>>(P->ntarg, trg[0], trg[1], trg[2], P->nptso, src[0], src[1],
-                                                  src[2], src[3], src[4], src[5], P->opatch, q, m, ldstr,
-                                                  P->row_ptr, P->near_sorted, kr, ki, th2, chunk, part, ldpart);
+#define FMM_FAR(MK, CKV)                                                                                       \
+  k_farsum<T, MK, CKV, NRHS><<<grid, TPB, 0, s>>>(P->ntarg, trg[0], trg[1], trg[2], P->nptso, src[0], src[1], src[2], \
+                                                  src[3], src[4], src[5], P->ixyzso, q, m, ldstr, P->row_ptr,       \
+                                                  P->near_sorted, kr, ki, th2, chunk, part, ldpart)
+  const bool ck = ki != (T)0;
+  if (mask) {
+    if (ck) FMM_FAR(true, true);
+    else FMM_FAR(true, false);
+  } else {
+    if (ck) FMM_FAR(false, true);
+    else FMM_FAR(false, false);
+  }
+#undef FMM_FAR
   *nsplit_out = nsplit;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
