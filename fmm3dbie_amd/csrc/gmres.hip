@@ -8,6 +8,7 @@
 // T = double (FP64 solve) or float (the FP32 inner solve of iterative refinement).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -142,6 +143,14 @@ __global__ void k_add_masked(int64_t n, C2<double>* __restrict__ x, const C2<T>*
   }
 }
 
+// y_r = V_r column k (the Krylov vector of each system, gathered for the batched apply)
+template <typename T>
+__global__ void k_gather(int64_t n, const C2<T>* __restrict__ V, int64_t ldvr, C2<T>* __restrict__ y) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (j < n) y[r * n + j] = V[r * ldvr + j];
+}
+
 inline dim3 g2(int64_t n, int r) { return dim3((unsigned)((n + 255) / 256), (unsigned)r); }
 
 #define GCHK(x)                                                                    \
@@ -168,9 +177,12 @@ int gmres_impl(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, voi
   GCHK(hipMalloc((void**)&V, sizeof(C2<T>) * ldvr * nrhs));
   GCHK(hipMalloc((void**)&W, sizeof(C2<T>) * N * nrhs));
   GCHK(hipMalloc((void**)&KX, sizeof(C2<T>) * N * nrhs));
-  GCHK(hipMalloc((void**)&dh, sizeof(double) * 2 * (numit + 2) * nrhs * 2));
-  double* dsc = dh + 2 * (numit + 2) * nrhs;
-  std::vector<double> hh(2 * (numit + 2) * nrhs), hh2(2 * (numit + 2) * nrhs);
+  // dot products of the two Gram-Schmidt passes and the norm, each 2 (numit + 2) nrhs
+  // doubles, read back together once per iteration; then the scale factors
+  const int64_t LH = 2 * (int64_t)(numit + 2) * nrhs;
+  GCHK(hipMalloc((void**)&dh, sizeof(double) * (3 * LH + 2 * nrhs)));
+  double* dsc = dh + 3 * LH;
+  std::vector<double> hh(LH), hh2(LH), hh3(3 * LH);
   // rb = ||rhs||_w
   k_dots<T><<<dim3(1, nrhs), 256, 0, s>>>(N, 1, rhs, N, N, rhs, N, wts, dh, numit + 2);
   GCHK(hipMemcpyAsync(hh.data(), dh, sizeof(double) * 2 * (numit + 2) * nrhs, hipMemcpyDeviceToHost, s));
@@ -193,29 +205,31 @@ int gmres_impl(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, voi
   int it = 0;
   for (it = 1; it <= numit; it++) {
     // W = K V_{it-1}  (all systems in one batched apply; V_{it-1} of system r at r*ldvr)
-    for (int r = 0; r < nrhs; r++)
-      GCHK(hipMemcpyAsync(KX + r * N, V + r * ldvr + (it - 1) * ldv, sizeof(C2<T>) * N, hipMemcpyDeviceToDevice,
-                          s));
-    int rc = apply_plan_run(P, prec, prec == 32 ? APPLY_MASKED : mode, KX, W, nrhs, s, nullptr);
+    const C2<T>* vin = V + (it - 1) * ldv;
+    if (nrhs > 1) {
+      k_gather<T><<<g2(N, nrhs), 256, 0, s>>>(N, vin, ldvr, KX);
+      vin = KX;
+    }
+    int rc = apply_plan_run(P, prec, prec == 32 ? APPLY_MASKED : mode, vin, W, nrhs, s, nullptr);
     if (rc) return rc;
     napply++;
-    // CGS2
-    std::vector<double> htot(2 * (numit + 2) * nrhs, 0.0);
+    // CGS2 (two passes, each on the device with its own dot products), then the norm;
+    // one read-back per iteration
     for (int pass = 0; pass < 2; pass++) {
-      k_dots<T><<<dim3(it, nrhs), 256, 0, s>>>(N, it, V, ldv, ldvr, W, N, wts, dh, numit + 2);
-      k_sub<T><<<g2(N, nrhs), 256, 0, s>>>(N, it, V, ldv, ldvr, W, N, dh, numit + 2);
-      GCHK(hipMemcpyAsync(hh.data(), dh, sizeof(double) * 2 * (numit + 2) * nrhs, hipMemcpyDeviceToHost, s));
-      GCHK(hipStreamSynchronize(s));
-      for (int r = 0; r < nrhs; r++)
-        for (int k = 0; k < it; k++) {
-          htot[2 * (r * (numit + 2) + k)] += hh[2 * (r * (numit + 2) + k)];
-          htot[2 * (r * (numit + 2) + k) + 1] += hh[2 * (r * (numit + 2) + k) + 1];
-        }
+      k_dots<T><<<dim3(it, nrhs), 256, 0, s>>>(N, it, V, ldv, ldvr, W, N, wts, dh + pass * LH, numit + 2);
+      k_sub<T><<<g2(N, nrhs), 256, 0, s>>>(N, it, V, ldv, ldvr, W, N, dh + pass * LH, numit + 2);
     }
-    // norms
-    k_dots<T><<<dim3(1, nrhs), 256, 0, s>>>(N, 1, W, N, N, W, N, wts, dh, numit + 2);
-    GCHK(hipMemcpyAsync(hh2.data(), dh, sizeof(double) * 2 * (numit + 2) * nrhs, hipMemcpyDeviceToHost, s));
+    k_dots<T><<<dim3(1, nrhs), 256, 0, s>>>(N, 1, W, N, N, W, N, wts, dh + 2 * LH, numit + 2);
+    GCHK(hipMemcpyAsync(hh3.data(), dh, sizeof(double) * 3 * LH, hipMemcpyDeviceToHost, s));
     GCHK(hipStreamSynchronize(s));
+    std::vector<double> htot(LH, 0.0);
+    for (int r = 0; r < nrhs; r++)
+      for (int k = 0; k < it; k++)
+        for (int c = 0; c < 2; c++) {
+          const int64_t o = 2 * (r * (numit + 2) + k) + c;
+          htot[o] = hh3[o] + hh3[LH + o];
+        }
+    std::copy(hh3.begin() + 2 * LH, hh3.end(), hh2.begin());
     for (int r = 0; r < nrhs; r++) {
       const double wn = std::sqrt(hh2[2 * (r * (numit + 2))]);
       scv[2 * r] = wn > 0 ? 1.0 / wn : 0.0;
