@@ -778,7 +778,8 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
 
 // Autotuner of the far-sum launch shape for (prec, nrhs): every right-hand-side group
 // size that divides nrhs (1..8 in FP64, 1..16 in FP32) and workgroup targets of 1024,
-// 2048, 4096 are timed (one warm-up, best of two) on unit densities; the fastest is kept
+// 4096, 8192, 16384, 32768 are timed (one warm-up launch, then one launch each) on unit
+// densities; the fastest is kept
 // in the plan and used by every later apply with the same (prec, nrhs).
 int apply_plan_tune(ApplyPlan* P, int prec, int nrhs, hipStream_t s, double* report, int nreport) {
   if (nrhs < 1 || nrhs > 64 || (prec != 32 && prec != 64)) return -6;
@@ -792,41 +793,41 @@ int apply_plan_tune(ApplyPlan* P, int prec, int nrhs, hipStream_t s, double* rep
     k_fill<float><<<nblk(2 * NO * nrhs, 256), 256, 0, s>>>(2 * NO * nrhs, P->q32, 1.0f);
     k_fill<float><<<nblk(2 * NO * nrhs, 256), 256, 0, s>>>(2 * NO * nrhs, P->m32, 1.0f);
   }
+  auto far = [&](int gsz, int64_t wg) {
+    int64_t nsplit;
+    return prec == 64 ? dispatch_far<double>(P, true, gsz, wg, (const V2<double>*)P->q64, (const V2<double>*)P->m64,
+                                             NO, nrhs, (V2<double>*)P->part64, &nsplit, s)
+                      : dispatch_far<float>(P, true, gsz, wg, (const V2<float>*)P->q32, (const V2<float>*)P->m32, NO,
+                                            nrhs, (V2<float>*)P->part32, &nsplit, s);
+  };
   hipEvent_t e0, e1;
   ACHK(hipEventCreate(&e0));
   ACHK(hipEventCreate(&e1));
   const int groups[5] = {1, 2, 4, 8, 16};
-  const int64_t wgs[3] = {1024, 2048, 4096};
+  const int64_t wgs[4] = {4096, 8192, 16384, 32768};
   float best = 1e30f;
   int bg = 1, nrep = 0;
   int64_t bw = P->far_wg_target;
+  // one warm-up launch, then one timed launch per shape
+  int rc = far(1, P->far_wg_target);
+  if (rc) return rc;
   for (int gsz : groups) {
     if (nrhs % gsz != 0 || (prec == 64 && gsz > 8)) continue;
     for (int64_t wg : wgs) {
-      float tbest = 1e30f;
-      for (int rep = 0; rep < 3; rep++) {
-        int64_t nsplit;
-        ACHK(hipEventRecord(e0, s));
-        int rc = prec == 64 ? dispatch_far<double>(P, true, gsz, wg, (const V2<double>*)P->q64,
-                                                   (const V2<double>*)P->m64, NO, nrhs, (V2<double>*)P->part64,
-                                                   &nsplit, s)
-                            : dispatch_far<float>(P, true, gsz, wg, (const V2<float>*)P->q32,
-                                                  (const V2<float>*)P->m32, NO, nrhs, (V2<float>*)P->part32, &nsplit,
-                                                  s);
-        if (rc) return rc;
-        ACHK(hipEventRecord(e1, s));
-        ACHK(hipEventSynchronize(e1));
-        float ms;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (rep > 0 && ms < tbest) tbest = ms;
-      }
+      ACHK(hipEventRecord(e0, s));
+      rc = far(gsz, wg);
+      if (rc) return rc;
+      ACHK(hipEventRecord(e1, s));
+      ACHK(hipEventSynchronize(e1));
+      float ms;
+      (void)hipEventElapsedTime(&ms, e0, e1);
       if (report && nrep + 3 <= nreport) {
         report[nrep++] = gsz;
         report[nrep++] = (double)wg;
-        report[nrep++] = tbest;
+        report[nrep++] = ms;
       }
-      if (tbest < best) {
-        best = tbest;
+      if (ms < best) {
+        best = ms;
         bg = gsz;
         bw = wg;
       }

@@ -286,8 +286,8 @@ int fmm3dbie_apply_plan_apply(void *plan, int prec, int mode, const void *sigma,
 int fmm3dbie_apply_plan_info(void *plan, int64_t *out4);
 void fmm3dbie_apply_plan_destroy(void *plan);
 /* Autotuner of the all-pairs far-sum launch (no reference counterpart): times every
- * right-hand-side group size dividing nrhs (FP64 1..8, FP32 1..16) and three workgroup
- * targets, keeps the fastest for later applies with the same (prec, nrhs); report gets
+ * right-hand-side group size dividing nrhs (FP64 1..8, FP32 1..16) and four workgroup
+ * targets (4096..32768), keeps the fastest for later applies with the same (prec, nrhs); report gets
  * (group, workgroup target, ms) triples (nreport doubles).  Returns the number timed. */
 int fmm3dbie_apply_plan_tune(void *plan, int prec, int nrhs, void *stream, double *report, int nreport);
 

@@ -34,12 +34,12 @@ def test_batched_apply_and_tuning(cuda):
     for mode in (0, 1):
         assert _rel(plan.apply(dS, 64, mode).cpu().numpy(), single) < 1e-12
     rep64 = plan.tune(64, 64)
-    assert len(rep64) == 4 * 3  # groups 1, 2, 4, 8 x three workgroup targets
+    assert len(rep64) == 4 * 4  # groups 1, 2, 4, 8 x four workgroup targets
     assert _rel(plan.apply(dS, 64, 1).cpu().numpy(), single) < 1e-12
     S32 = dS.to(torch.complex64)
     before = plan.apply(S32, 32, 1).cpu().numpy()
     rep32 = plan.tune(32, 64)
-    assert len(rep32) == 5 * 3
+    assert len(rep32) == 5 * 4
     after = plan.apply(S32, 32, 1).cpu().numpy()
     for r in range(64):
         assert _rel(after[r], single[r]) < 1e-5
