@@ -333,6 +333,34 @@ __global__ void k_reduce_csr(int64_t ntarg, int nrhs, int nsplit, const V2<T>* _
   }
 }
 
+// Sum of the far-sum partials over the source splits, in place into split 0: a workgroup
+// of 256 threads takes 64 targets, thread (t, g) adds splits g, g + 4, ... of target t
+// (loads coalesced over t) and the four groups meet in LDS.
+template <typename T>
+__global__ void __launch_bounds__(256) k_split_sum(int64_t ntarg, int nsplit, V2<T>* __restrict__ part,
+                                                   int64_t ldpart) {
+  __shared__ V2<T> sh[4][64];
+  const int t = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int64_t i = blockIdx.x * (int64_t)64 + t;
+  const int64_t r = blockIdx.y;
+  T sr = 0, si = 0;
+  if (i < ntarg) {
+    const V2<T>* pp = part + r * ntarg + i;
+#pragma unroll 4
+    for (int sp = g; sp < nsplit; sp += 4) {
+      const V2<T> v = pp[(int64_t)sp * ldpart];
+      sr += v.x;
+      si += v.y;
+    }
+  }
+  sh[g][t] = {sr, si};
+  __syncthreads();
+  if (g == 0 && i < ntarg) {
+    const V2<T> a = sh[1][t], b = sh[2][t], c = sh[3][t];
+    part[r * ntarg + i] = {(sr + a.x) + (b.x + c.x), (si + a.y) + (b.y + c.y)};
+  }
+}
+
 // constant fill (tuning densities)
 template <typename T>
 __global__ void k_fill(int64_t n, T* __restrict__ a, T v) {
@@ -677,6 +705,12 @@ static int far_and_csr(ApplyPlan* P, int prec, bool mask, const V2<T>* q, const 
     if (rc) return rc;
   }
   if (ev_far) ACHK(hipEventRecord(ev_far, s));
+  if (nsplit > 4) {
+    // many source splits: one coalesced pass over the partials before the per-target CSR
+    k_split_sum<T><<<dim3((unsigned)((N + 63) / 64), (unsigned)nrhs), 256, 0, s>>>(N, (int)nsplit, part,
+                                                                                  (int64_t)nrhs * N);
+    nsplit = 1;
+  }
   if (P->uniform_p > 0) {
 #define FMM_CSR(PC)                                                                                             \
   if (nrhs == 1)                                                                                               \

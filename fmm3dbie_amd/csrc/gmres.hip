@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "apply.hpp"
+#include "common.hpp"
 #include "gmres.hpp"
 
 namespace {
@@ -24,16 +25,19 @@ struct C2 {
   T x, y;
 };
 
-// h[r*ldh + k] = sum_j w_j conj(V_k[j]) x_r[j]  for k < nk; one block per (k, r)
+// Partial weighted dots: hp[((r*ldh + k)*S + z)] = sum over rows j of slice z of
+// w_j conj(V_k[j]) x_r[j]; one block per (k, r, slice z) so that a Krylov basis of a few
+// vectors still spreads over the GPU.
 template <typename T>
-__global__ void k_dots(int64_t n, int nk, const C2<T>* __restrict__ V, int64_t ldv, int64_t ldvr,
+__global__ void k_dots(int64_t n, int64_t chunk, const C2<T>* __restrict__ V, int64_t ldv, int64_t ldvr,
                        const C2<T>* __restrict__ x, int64_t ldx, const double* __restrict__ w,
-                       double* __restrict__ h, int ldh) {
-  const int k = blockIdx.x, r = blockIdx.y;
+                       double* __restrict__ hp, int ldh) {
+  const int k = blockIdx.x, r = blockIdx.y, z = blockIdx.z, S = gridDim.z;
   const C2<T>* vk = V + r * ldvr + k * ldv;
   const C2<T>* xr = x + r * ldx;
+  const int64_t j0 = z * chunk, j1 = j0 + chunk < n ? j0 + chunk : n;
   double sr = 0.0, si = 0.0;
-  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+  for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
     const double ar = vk[j].x, ai = -(double)vk[j].y, br = xr[j].x, bi = xr[j].y;
     sr += w[j] * (ar * br - ai * bi);
     si += w[j] * (ar * bi + ai * br);
@@ -42,17 +46,50 @@ __global__ void k_dots(int64_t n, int nk, const C2<T>* __restrict__ V, int64_t l
   shr[threadIdx.x] = sr;
   shi[threadIdx.x] = si;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      shr[threadIdx.x] += shr[threadIdx.x + s];
-      shi[threadIdx.x] += shi[threadIdx.x + s];
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      shr[threadIdx.x] += shr[threadIdx.x + st];
+      shi[threadIdx.x] += shi[threadIdx.x + st];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    h[2 * (r * ldh + k)] = shr[0];
-    h[2 * (r * ldh + k) + 1] = shi[0];
+    const int64_t o = ((int64_t)(r * ldh + k) * S + z) * 2;
+    hp[o] = shr[0];
+    hp[o + 1] = shi[0];
   }
+}
+
+// h[r*ldh + k] = sum over the S slices (one wavefront per (k, r))
+__global__ void k_dots_sum(int S, const double* __restrict__ hp, double* __restrict__ h, int ldh) {
+  const int k = blockIdx.x, r = blockIdx.y, lane = threadIdx.x;
+  const int64_t o = (int64_t)(r * ldh + k) * S;
+  double sr = 0.0, si = 0.0;
+  for (int z = lane; z < S; z += 64) {
+    sr += hp[(o + z) * 2];
+    si += hp[(o + z) * 2 + 1];
+  }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  if (lane == 0) {
+    h[2 * (r * ldh + k)] = sr;
+    h[2 * (r * ldh + k) + 1] = si;
+  }
+}
+
+constexpr int64_t DOT_ROWS = 4096;  // rows per dot-product slice
+constexpr int DOT_SMAX = 64;
+
+// h[r*ldh + k] = sum_j w_j conj(V_k[j]) x_r[j]  for k < nk (hp: scratch of
+// 2 * ldh * nrhs * DOT_SMAX doubles)
+template <typename T>
+void dots(int64_t n, int nk, const C2<T>* V, int64_t ldv, int64_t ldvr, const C2<T>* x, int64_t ldx,
+          const double* w, double* h, int ldh, int nrhs, double* hp, hipStream_t s) {
+  int S = (int)((n + DOT_ROWS - 1) / DOT_ROWS);
+  S = S < 1 ? 1 : (S > DOT_SMAX ? DOT_SMAX : S);
+  const int64_t chunk = (n + S - 1) / S;
+  k_dots<T><<<dim3(nk, nrhs, S), 256, 0, s>>>(n, chunk, V, ldv, ldvr, x, ldx, w, hp, ldh);
+  k_dots_sum<<<dim3(nk, nrhs), 64, 0, s>>>(S, hp, h, ldh);
 }
 
 // x_r -= sum_k h[r,k] V_k
@@ -180,11 +217,12 @@ int gmres_impl(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, voi
   // dot products of the two Gram-Schmidt passes and the norm, each 2 (numit + 2) nrhs
   // doubles, read back together once per iteration; then the scale factors
   const int64_t LH = 2 * (int64_t)(numit + 2) * nrhs;
-  GCHK(hipMalloc((void**)&dh, sizeof(double) * (3 * LH + 2 * nrhs)));
+  GCHK(hipMalloc((void**)&dh, sizeof(double) * (3 * LH + 2 * nrhs + LH * DOT_SMAX)));
   double* dsc = dh + 3 * LH;
+  double* dhp = dsc + 2 * nrhs;  // partial dot products of the row slices
   std::vector<double> hh(LH), hh2(LH), hh3(3 * LH);
   // rb = ||rhs||_w
-  k_dots<T><<<dim3(1, nrhs), 256, 0, s>>>(N, 1, rhs, N, N, rhs, N, wts, dh, numit + 2);
+  dots<T>(N, 1, rhs, N, N, rhs, N, wts, dh, numit + 2, nrhs, dhp, s);
   GCHK(hipMemcpyAsync(hh.data(), dh, sizeof(double) * 2 * (numit + 2) * nrhs, hipMemcpyDeviceToHost, s));
   GCHK(hipStreamSynchronize(s));
   std::vector<double> rb(nrhs);
@@ -216,10 +254,10 @@ int gmres_impl(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, voi
     // CGS2 (two passes, each on the device with its own dot products), then the norm;
     // one read-back per iteration
     for (int pass = 0; pass < 2; pass++) {
-      k_dots<T><<<dim3(it, nrhs), 256, 0, s>>>(N, it, V, ldv, ldvr, W, N, wts, dh + pass * LH, numit + 2);
+      dots<T>(N, it, V, ldv, ldvr, W, N, wts, dh + pass * LH, numit + 2, nrhs, dhp, s);
       k_sub<T><<<g2(N, nrhs), 256, 0, s>>>(N, it, V, ldv, ldvr, W, N, dh + pass * LH, numit + 2);
     }
-    k_dots<T><<<dim3(1, nrhs), 256, 0, s>>>(N, 1, W, N, N, W, N, wts, dh + 2 * LH, numit + 2);
+    dots<T>(N, 1, W, N, N, W, N, wts, dh + 2 * LH, numit + 2, nrhs, dhp, s);
     GCHK(hipMemcpyAsync(hh3.data(), dh, sizeof(double) * 3 * LH, hipMemcpyDeviceToHost, s));
     GCHK(hipStreamSynchronize(s));
     std::vector<double> htot(LH, 0.0);
@@ -297,7 +335,7 @@ int gmres_impl(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, voi
     if (rc) return rc;
     napply++;
     k_resid<T><<<g2(N, nrhs), 256, 0, s>>>(N, rhs, soln, KX, zid[0], zid[1], W, N);
-    k_dots<T><<<dim3(1, nrhs), 256, 0, s>>>(N, 1, W, N, N, W, N, wts, dh, numit + 2);
+    dots<T>(N, 1, W, N, N, W, N, wts, dh, numit + 2, nrhs, dhp, s);
     GCHK(hipMemcpyAsync(hh.data(), dh, sizeof(double) * 2 * (numit + 2) * nrhs, hipMemcpyDeviceToHost, s));
     GCHK(hipStreamSynchronize(s));
     for (int r = 0; r < nrhs; r++) rres[r] = std::sqrt(hh[2 * (r * (numit + 2))]) / rb[r];
@@ -344,11 +382,12 @@ int gmres_ir_device(ApplyPlan* P, int mode, const double* zid, const void* rhs_v
   GCHK(hipMalloc((void**)&kx, sizeof(C2<double>) * N * nrhs));
   GCHK(hipMalloc((void**)&r32, sizeof(C2<float>) * N * nrhs));
   GCHK(hipMalloc((void**)&c32, sizeof(C2<float>) * N * nrhs));
-  GCHK(hipMalloc((void**)&dh, sizeof(double) * 2 * nrhs * 2));
+  GCHK(hipMalloc((void**)&dh, sizeof(double) * 4 * nrhs * (1 + DOT_SMAX)));
+  double* dhp = dh + 4 * nrhs;  // partial dot products of the row slices
   GCHK(hipMemsetAsync(x, 0, sizeof(C2<double>) * N * nrhs, s));
   GCHK(hipMemcpyAsync(r64, rhs, sizeof(C2<double>) * N * nrhs, hipMemcpyDeviceToDevice, s));
   std::vector<double> hb(2 * nrhs * 2), hr(2 * nrhs * 2);
-  k_dots<double><<<dim3(1, nrhs), 256, 0, s>>>(N, 1, rhs, N, N, rhs, N, wts, dh, 2);
+  dots<double>(N, 1, rhs, N, N, rhs, N, wts, dh, 2, nrhs, dhp, s);
   GCHK(hipMemcpyAsync(hb.data(), dh, sizeof(double) * 4 * nrhs, hipMemcpyDeviceToHost, s));
   GCHK(hipStreamSynchronize(s));
   std::vector<int> nit(nrhs), upd(nrhs, 1);
@@ -379,7 +418,7 @@ int gmres_ir_device(ApplyPlan* P, int mode, const double* zid, const void* rhs_v
     if (rc) return rc;
     if (gs) gs->napply64++;
     k_resid<double><<<g2(N, nrhs), 256, 0, s>>>(N, rhs, x, kx, zid[0], zid[1], r64, N);
-    k_dots<double><<<dim3(1, nrhs), 256, 0, s>>>(N, 1, r64, N, N, r64, N, wts, dh, 2);
+    dots<double>(N, 1, r64, N, N, r64, N, wts, dh, 2, nrhs, dhp, s);
     GCHK(hipMemcpyAsync(hr.data(), dh, sizeof(double) * 4 * nrhs, hipMemcpyDeviceToHost, s));
     GCHK(hipStreamSynchronize(s));
     worst = 0.0;
