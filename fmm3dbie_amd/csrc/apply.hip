@@ -66,9 +66,12 @@ __global__ void k_oversample(int64_t nptso, const int* __restrict__ opatch, cons
   }
 }
 
-// e^{ikr}/r and the dipole factor; T = double or float; CK: complex wavenumber
-template <typename T, bool CK>
-__device__ __forceinline__ void pair_kernel(T dx, T dy, T dz, T nx, T ny, T nz, T kr, T ki, T& er, T& ei,
+// e^{ikr}/r and the dipole factor; T = double or float.  GEN: complex wavenumber, or (FP32)
+// phases beyond the hardware sine's range, reduced to one revolution first; otherwise the
+// FP32 phase k r / (2 pi) = r * krev goes to the hardware sine and cosine directly
+// (valid for |k r| / (2 pi) < 256, which the plan checks against its bounding box).
+template <typename T, bool GEN>
+__device__ __forceinline__ void pair_kernel(T dx, T dy, T dz, T nx, T ny, T nz, T kr, T krev, T ki, T& er, T& ei,
                                             T& tdr, T& tdi, T& r2out) {
   const T r2 = dx * dx + dy * dy + dz * dz;
   r2out = r2;
@@ -83,21 +86,27 @@ __device__ __forceinline__ void pair_kernel(T dx, T dy, T dz, T nx, T ny, T nz, 
   const T d = dx * nx + dy * ny + dz * nz;
   T s, c;
   if constexpr (sizeof(T) == 4) {
-    __sincosf(kr * r, &s, &c);
+    T rev = r * krev;
+    if constexpr (GEN) rev -= rintf(rev);
+    s = __builtin_amdgcn_sinf(rev);
+    c = __builtin_amdgcn_cosf(rev);
   } else {
+    (void)krev;
     sincos(kr * r, &s, &c);
   }
   T mag = rinv;
-  if constexpr (CK) {
+  if constexpr (GEN) {
     if constexpr (sizeof(T) == 4) mag *= __expf(-ki * r);
     else mag *= exp(-ki * r);
   }
   er = c * mag;
   ei = s * mag;
-  // dipole factor t*(1 - i k r), t = d/r^2, k = kr + i ki: (1 + ki r) - i kr r
-  const T t = d * rinv * rinv;
-  tdr = CK ? t * ((T)1 + ki * r) : t;
-  tdi = -t * kr * r;
+  // dipole factor t (1 - i k r), t = d / r^2, k = kr + i ki: (1 + ki r) - i kr r, and
+  // t kr r = kr d / r
+  const T u = d * rinv;
+  const T t = u * rinv;
+  tdr = GEN ? t * ((T)1 + ki * r) : t;
+  tdi = -kr * u;
 }
 
 // Tiled far sum.  pot[i] (+)= sum_j [q_j + m_j * t_ij (1-ikr)] e^{ikr}/r / (4 pi)
@@ -105,7 +114,7 @@ __device__ __forceinline__ void pair_kernel(T dx, T dy, T dz, T nx, T ny, T nz, 
 // by patch, so each thread keeps the source interval [ea, eb) of the next near patch
 // (patch p owns sources ixyzso[p]-1 .. ixyzso[p+1]-2) and moves it forward when the tile
 // passes eb; the pair loop itself has no branches (a skipped pair adds an exact zero).
-template <typename T, bool MASK, bool CK, int NRHS>
+template <typename T, bool MASK, bool GEN, int NRHS>
 __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restrict__ tx, const T* __restrict__ ty,
                                                 const T* __restrict__ tz, int64_t nsrc,
                                                 const T* __restrict__ sx, const T* __restrict__ sy,
@@ -114,7 +123,7 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
                                                 const int64_t* __restrict__ ixyzso, const V2<T>* __restrict__ qstr,
                                                 const V2<T>* __restrict__ mstr, int64_t ldstr,
                                                 const int64_t* __restrict__ row_ptr,
-                                                const int* __restrict__ near_sorted, T kr, T ki, T thresh2,
+                                                const int* __restrict__ near_sorted, T kr, T krev, T ki, T thresh2,
                                                 int64_t chunk, V2<T>* __restrict__ part, int64_t ldpart) {
   __shared__ T lx[TPB], ly[TPB], lz[TPB], lnx[TPB], lny[TPB], lnz[TPB];
   __shared__ V2<T> lq[NRHS][TPB], lm[NRHS][TPB];
@@ -171,8 +180,10 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
       const int64_t jk = base + k;
       if (MASK && jk >= eb) advance(jk);
       T er, ei, tdr, tdi, r2;
-      pair_kernel<T, CK>(x - lx[k], y - ly[k], z - lz[k], lnx[k], lny[k], lnz[k], kr, ki, er, ei, tdr, tdi, r2);
-      const bool keep = (!MASK || jk < ea) && r2 > thresh2;
+      pair_kernel<T, GEN>(x - lx[k], y - ly[k], z - lz[k], lnx[k], lny[k], lnz[k], kr, krev, ki, er, ei, tdr, tdi,
+                          r2);
+      // with the mask, sources within thresh of the target belong to its near patches
+      const bool keep = MASK ? jk < ea : r2 > thresh2;
       constexpr bool SEL_FACTORS = NRHS > 2;  // zero the pair's factors once, not each sum
       if (SEL_FACTORS) {
         er = keep ? er : (T)0;
@@ -236,7 +247,8 @@ __global__ void k_near_sub(int64_t ntarg, const double* __restrict__ tx, const d
       const int64_t ip = col_ind[jj] - 1;
       for (int64_t j = ixyzso[ip] - 1 + lane; j < ixyzso[ip + 1] - 1; j += 64) {
         double er, ei, tdr, tdi, r2;
-        pair_kernel<double, true>(x - sx[j], y - sy[j], z - sz[j], snx[j], sny[j], snz[j], kr, ki, er, ei, tdr, tdi, r2);
+        pair_kernel<double, true>(x - sx[j], y - sy[j], z - sz[j], snx[j], sny[j], snz[j], kr, 0.0, ki, er, ei, tdr,
+                                  tdi, r2);
         if (r2 <= thresh2) continue;
         const V2<double> q = qstr[r * ldstr + j], m = mstr[r * ldstr + j];
         const double ir = q.x + m.x * tdr - m.y * tdi;
@@ -499,6 +511,11 @@ int apply_plan_build(ApplyPlan* P, const ApplyPlanInput& in, hipStream_t s) {
   if (mx[1] - mn[1] > bs) bs = mx[1] - mn[1];
   if (mx[2] - mn[2] > bs) bs = mx[2] - mn[2];
   P->thresh = std::ldexp(1.0, -51) * bs;
+  // largest FP32 phase |k| r / (2 pi) over the box diagonal: the hardware sine takes up to
+  // 256 revolutions; beyond 200 the FP32 far sum reduces the phase first
+  const double diag = std::sqrt((mx[0] - mn[0]) * (mx[0] - mn[0]) + (mx[1] - mn[1]) * (mx[1] - mn[1]) +
+                                (mx[2] - mn[2]) * (mx[2] - mn[2]));
+  P->phase_small = !in.helm || std::fabs(in.zk[0]) * diag * 0.15915494309189533577 < 200.0;
 
   P->ixyzs = own<int64_t>(P, npat + 1);
   P->ixyzso = own<int64_t>(P, npat + 1);
@@ -630,6 +647,7 @@ static int launch_far(ApplyPlan* P, bool mask, int64_t wg_target, const V2<T>* q
     trg = (const T* const*)P->trg32;
   }
   const T kr = (T)(P->helm ? P->zk[0] : 0.0), ki = (T)(P->helm ? P->zk[1] : 0.0);
+  const T krev = (T)(P->helm ? P->zk[0] * 0.15915494309189533577 : 0.0);  // k / (2 pi)
   const T th2 = (T)(P->thresh * P->thresh);
   const unsigned g = nblk(P->ntarg, TPB);
   const int ngrp = nrhs / NRHS;
@@ -646,8 +664,8 @@ static int launch_far(ApplyPlan* P, bool mask, int64_t wg_target, const V2<T>* q
 #define FMM_FAR(MK, CKV)                                                                                       \
   k_farsum<T, MK, CKV, NRHS><<<grid, TPB, 0, s>>>(P->ntarg, trg[0], trg[1], trg[2], P->nptso, src[0], src[1], src[2], \
                                                   src[3], src[4], src[5], P->ixyzso, q, m, ldstr, P->row_ptr,       \
-                                                  P->near_sorted, kr, ki, th2, chunk, part, ldpart)
-  const bool ck = ki != (T)0;
+                                                  P->near_sorted, kr, krev, ki, th2, chunk, part, ldpart)
+  const bool ck = ki != (T)0 || (sizeof(T) == 4 && !P->phase_small);
   if (mask) {
     if (ck) FMM_FAR(true, true);
     else FMM_FAR(true, false);

@@ -34,6 +34,7 @@ struct ApplyPlan {
   int64_t ntarg = 0;  // targets (rows of the near list); npts for the on-surface operator
   double zk[2] = {0, 0}, alpha[2] = {0, 0}, beta[2] = {0, 0};
   double thresh = 0;
+  bool phase_small = true;  // FP32 phases within the hardware sine's range (no reduction)
   int uniform_p = 0;
   int64_t *ixyzs = nullptr, *ixyzso = nullptr, *row_ptr = nullptr, *col_ind = nullptr, *iquad = nullptr,
           *ioff = nullptr;
