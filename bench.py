@@ -382,7 +382,8 @@ def learned_mixed_solve(dev):
         w = nq.run(selector=params)
         plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"], P["ixyzso"],
                                P["srcover"], P["whtsover"])
-        plan.tune(32, 1)
+        # one right-hand side: the default far-sum shape (tuning costs more than it saves
+        # in a single solve)
         t0 = time.perf_counter()
         x, outer, inner, rr, n32, n64 = layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=5e-7, tol=1e-6)
         torch.cuda.synchronize()

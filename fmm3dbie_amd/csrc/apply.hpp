@@ -54,8 +54,9 @@ struct ApplyPlan {
   double *q64 = nullptr, *m64 = nullptr, *sig64c = nullptr, *pot64c = nullptr, *part64 = nullptr;
   float *q32 = nullptr, *m32 = nullptr, *part32 = nullptr;
   // far-sum launch shape: target workgroups x source splits >= far_wg_target (many short
-  // workgroups: 64 per CU balance the 256 CUs better than one long round)
-  int64_t far_wg_target = 16384;
+  // workgroups, ~128 per CU, balance the 256 CUs better than one long round; the partial
+  // sums of the splits are added by one coalesced pass)
+  int64_t far_wg_target = 32768;
   int far_split_cap = 8;
   // autotuned far-sum shape per (prec * 1000 + nrhs): (RHS group size, workgroup target)
   std::map<int, std::pair<int, int64_t>> tuned;
