@@ -791,15 +791,15 @@ __global__ void __launch_bounds__(64) k_shallow(int npairs, const int2* __restri
   const double* cp = cache + (size_t)ipa * CACHE_STRIDE * nn;
   const dbl4 z4 = {0.0, 0.0, 0.0, 0.0};
   // LDS accumulators (each lane only touches its own elements, so no barrier is needed
-  // until the end): sM = sum of the children, sS = sum of the grandchildren, cM / cS =
-  // the current child and the sum of its children
-  __shared__ double sM[16 * RW], sS[DEPTH == 2 ? 16 * RW : 1], cM[DEPTH == 2 ? 16 * RW : 1],
-      cS[DEPTH == 2 ? 16 * RW : 1];
+  // until the end): sM = sum of the children, sS = sum of the grandchildren.  The sum of
+  // the current child's children is kept in registers (cS).
+  __shared__ double sM[16 * RW], sS[DEPTH == 2 ? 16 * RW : 1];
   auto el = [&](int r, int a, int ct) { return (ka + 4 * r) * RW + a * CT * 16 + ct * 16 + ia; };
   double cmax[4] = {0.0, 0.0, 0.0, 0.0}, e0[4] = {0.0, 0.0, 0.0, 0.0};
-  // triangles in order: DEPTH 1: children 1..4, root; DEPTH 2: each child followed by
-  // its four children (5 + 4 (c - 1) + g), then the root.  One tile call site keeps the
-  // kernel evaluation inlined once.
+  dbl4 cS[NPART][CT];
+  // triangles in order: DEPTH 1: children 1..4, root; DEPTH 2: the four children of child
+  // c (5 + 4 (c - 1) + g) followed by child c, for c = 1..4, then the root.  One tile call
+  // site keeps the kernel evaluation inlined once.
   constexpr int NTASK = DEPTH == 1 ? 5 : 21;
 #pragma unroll 1
   for (int t = 0; t < NTASK; t++) {
@@ -811,7 +811,8 @@ __global__ void __launch_bounds__(64) k_shallow(int npairs, const int2* __restri
       c = 1 + t;
     } else {
       c = 1 + t / 5;
-      g = t % 5 - 1;
+      g = t % 5;  // 0..3: grandchild, 4: the child itself
+      if (g == 4) g = -1;
       tri = g < 0 ? c : 5 + 4 * (c - 1) + g;
     }
     dbl4 T[NPART][CT];
@@ -830,8 +831,26 @@ __global__ void __launch_bounds__(64) k_shallow(int npairs, const int2* __restri
 #pragma unroll
           for (int ct = 0; ct < CT; ct++) U[a][ct][r] = sM[el(r, a, ct)];
       tile_err<NPART, CT>(U, T, e0);
-    } else if (g < 0) {
-      // a child of the root
+    } else if (g >= 0) {
+      // a grandchild: accumulate the sum of child c's children
+#pragma unroll
+      for (int a = 0; a < NPART; a++)
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) {
+          if (g == 0) cS[a][ct] = T[a][ct];
+          else
+#pragma unroll
+            for (int r = 0; r < 4; r++) cS[a][ct][r] += T[a][ct][r];
+        }
+    } else {
+      // a child of the root (after its children for DEPTH 2): test it, add it to sM and its
+      // children to sS
+      if (DEPTH == 2) {
+        double ec[4];
+        tile_err<NPART, CT>(cS, T, ec);
+#pragma unroll
+        for (int r = 0; r < 4; r++) cmax[r] = ec[r] > cmax[r] || ec[r] != ec[r] ? ec[r] : cmax[r];
+      }
 #pragma unroll
       for (int r = 0; r < 4; r++)
 #pragma unroll
@@ -840,37 +859,8 @@ __global__ void __launch_bounds__(64) k_shallow(int npairs, const int2* __restri
           for (int ct = 0; ct < CT; ct++) {
             const int e = el(r, a, ct);
             sM[e] = (c == 1 ? 0.0 : sM[e]) + T[a][ct][r];
-            if (DEPTH == 2) cM[e] = T[a][ct][r];
+            if (DEPTH == 2) sS[e] = (c == 1 ? 0.0 : sS[e]) + cS[a][ct][r];
           }
-    } else {
-      // a grandchild: accumulate; after the fourth, test the child
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int a = 0; a < NPART; a++)
-#pragma unroll
-          for (int ct = 0; ct < CT; ct++) {
-            const int e = el(r, a, ct);
-            cS[e] = (g == 0 ? 0.0 : cS[e]) + T[a][ct][r];
-          }
-      if (g == 3) {
-        dbl4 U[NPART][CT], V[NPART][CT];
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-          for (int a = 0; a < NPART; a++)
-#pragma unroll
-            for (int ct = 0; ct < CT; ct++) {
-              const int e = el(r, a, ct);
-              U[a][ct][r] = cS[e];
-              V[a][ct][r] = cM[e];
-              sS[e] = (c == 1 ? 0.0 : sS[e]) + cS[e];
-            }
-        double ec[4];
-        tile_err<NPART, CT>(U, V, ec);
-#pragma unroll
-        for (int r = 0; r < 4; r++) cmax[r] = ec[r] > cmax[r] || ec[r] != ec[r] ? ec[r] : cmax[r];
-      }
     }
   }
   // decide per accumulator row; the accepted moments end up in sM
