@@ -25,6 +25,14 @@ PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix) FLOP/s
 PEAK_HBM = 8.0e12
 
 
+def far_flops_per_pair(m):
+    """Far sum, flops per (target, source) pair for m right-hand sides: 23 for the pair
+    (difference 3, r^2 5, 1/r 1, r 1, n.(x-y) 5, phase 1, sin and cos 2, e^{ikr}/r 2,
+    dipole factor 3) and 16 per density (q + m t(1-ikr): complex multiply-add 8; times
+    e^{ikr}/r: complex multiply 6; accumulate 2)."""
+    return 23.0 + 16.0 * m
+
+
 def workload(cfg):
     from fmm3dbie_amd import geometry as G
     if cfg == 0:
@@ -248,8 +256,8 @@ def main():
                         "speedup_mixed_vs_fp64": ts64 / tsir, "field_err_fp64": e64, "field_err_mixed": eir,
                         "mixed_vs_fp64_solution_rel": rel_sol,
                         "nptso": nptso,
-                        "far_fp64_tflops": npts_ * nptso * 44.0 / (t64[1] * 1e-3) / 1e12,
-                        "far_fp32_tflops": npts_ * nptso * 44.0 / (t32[1] * 1e-3) / 1e12,
+                        "far_fp64_tflops": npts_ * nptso * far_flops_per_pair(1) / (t64[1] * 1e-3) / 1e12,
+                        "far_fp32_tflops": npts_ * nptso * far_flops_per_pair(1) / (t32[1] * 1e-3) / 1e12,
                         # near correction (fused with the far reduction): wnear + CSR index arrays +
                         # row pointers + sigma and pot (the partial sums read are not counted)
                         "csr_fp64_gbs": (16.0 * P["nquad"] + 16.0 * P["nnz"] + 8.0 * (N + 1) + 32.0 * N)
@@ -307,11 +315,11 @@ def large_apply(g, pars, P, wnear, dev):
     tune = plan.tune(32, 16)
     plan.apply(S.to(torch.complex64), 32, 1, times=t32b)
     plan.close()
-    fl = npts * nptso * 44.0
+    fl = npts * nptso * far_flops_per_pair(1)
     return {"npts": npts, "nptso": nptso, "far_fp64_ms": t64[1], "far_fp32_ms": t32[1],
             "far_fp32_16rhs_ms": t32b[1], "far_fp64_tflops": fl / (t64[1] * 1e-3) / 1e12,
             "far_fp32_tflops": fl / (t32[1] * 1e-3) / 1e12,
-            "far_fp32_16rhs_tflops": npts * nptso * (20.0 + 24.0 * 16) / (t32b[1] * 1e-3) / 1e12,
+            "far_fp32_16rhs_tflops": npts * nptso * far_flops_per_pair(16) / (t32b[1] * 1e-3) / 1e12,
             "csr_fp64_ms": t64[2], "fp32_vs_fp64_rel": rel,
             "tuned": min(tune, key=lambda t: t[2]) if tune else None}
 
