@@ -940,59 +940,81 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
   Mom acc{0.0, 0.0};
   const int nn = nr * nt;
   const float inv_nr = 1.0f / (float)nr;
-  for (int e = 0; e < 3; e++) {
-    const double Ax = vx[e] - u0, Ay = vy[e] - v0;
-    const double Bx = vx[(e + 1) % 3] - u0, By = vy[(e + 1) % 3] - v0;
-    const double ax = l11 * Ax + l12 * Ay, ay = l22 * Ay;
-    const double bx = l11 * Bx + l12 * By, by = l22 * By;
-    const double len = sqrt((bx - ax) * (bx - ax) + (by - ay) * (by - ay));
-    const double ex = (bx - ax) / len, ey = (by - ay) / len;
-    const double ts = -(ax * ex + ay * ey);
-    const double fx = ax + ts * ex, fy = ay + ts * ey;
-    const double d = sqrt(fx * fx + fy * fy);
-    const double s0 = -ts, s1 = len - ts;
-    double segs[3];
-    int nseg = 0;
-    segs[nseg++] = s0;
-    if (s0 < 0.0 && s1 > 0.0) segs[nseg++] = 0.0;
-    segs[nseg++] = s1;
-    for (int sg = 0; sg + 1 < nseg; sg++) {
-      const double ta = asinh(segs[sg] / d), tb = asinh(segs[sg + 1] / d);
-      const double span = fabs(tb - ta) / tau_panel;
-      int npan = span < 64.0 ? (int)ceil(span) : 64;  // also maps NaN to 1 below
-      if (!(npan >= 1)) npan = 1;
-      for (int ipan = 0; ipan < npan; ipan++) {
-        const double t0 = ta + (tb - ta) * ipan / npan, t1 = ta + (tb - ta) * (ipan + 1) / npan;
-        for (int base = 0; base < nn; base += WAVE) {
-          const int k = base + lane;
-          if (k < nn) {
-            double* row = buf + lane * row_w(NP);
-            // k < 256: a float reciprocal gives the exact quotient
-            const int it = (int)(((float)k + 0.5f) * inv_nr), ir = k - it * nr;
-            const double tau = t0 + (t1 - t0) * g_self_gl.x[it];
-            // sinh and cosh from one exponential (only d * sinh enters, so the absolute
-            // error of the difference near tau = 0 is what matters)
-            const double et = exp(tau), eti = 1.0 / et;
-            const double sh = 0.5 * (et - eti), ch = 0.5 * (et + eti);
-            const double wtau = g_self_gl.w[it] * (t1 - t0);
-            const double px = fx + d * sh * ex, py = fy + d * sh * ey;
-            const double rho = g_self_gl.x[ir];
-            const double xi0 = rho * px, xi1 = rho * py;
-            const double dv = xi1 / l22, du = (xi0 - l12 * dv) / l11;
-            double g[12];
-            koorn_geom_row<N>(u0 + du, v0 + dv, sc, g, row + 2);
-            const double jac = geom_finish(g);
-            const double w = g_self_gl.w[ir] * wtau * rho * d * d * ch / detl * jac;
-            double vr, vi;
-            kern_eval<HELM>(tg, g, kp, vr, vi);
-            row[0] = vr * w;
-            row[1] = vi * w;
-          }
-          const int nrows = nn - base < WAVE ? nn - base : WAVE;
-          acc += column_sum<NP, HELM>(buf, nrows);
-        }
+  // The three triangles at the target, each split at the foot of the perpendicular: up to
+  // six segments of sinh-graded panels.  Their parameters go to LDS, and the nodes of all
+  // panels of all segments are taken 64 at a time (one pass per 64 nodes rather than per
+  // panel: a 100-node panel would leave 28 lanes of its second pass idle).
+  __shared__ double segd[6][8];  // fx, fy, ex, ey, d, ta, tb, npan
+  __shared__ int segc[7];        // first node of each segment; segc[nseg] = node count
+  if (lane == 0) {
+    int ns = 0, cnt = 0;
+    for (int e = 0; e < 3; e++) {
+      const double Ax = vx[e] - u0, Ay = vy[e] - v0;
+      const double Bx = vx[(e + 1) % 3] - u0, By = vy[(e + 1) % 3] - v0;
+      const double ax = l11 * Ax + l12 * Ay, ay = l22 * Ay;
+      const double bx = l11 * Bx + l12 * By, by = l22 * By;
+      const double len = sqrt((bx - ax) * (bx - ax) + (by - ay) * (by - ay));
+      const double ex = (bx - ax) / len, ey = (by - ay) / len;
+      const double ts = -(ax * ex + ay * ey);
+      const double fx = ax + ts * ex, fy = ay + ts * ey;
+      const double d = sqrt(fx * fx + fy * fy);
+      const double s0 = -ts, s1 = len - ts;
+      double segs[3];
+      int nseg = 0;
+      segs[nseg++] = s0;
+      if (s0 < 0.0 && s1 > 0.0) segs[nseg++] = 0.0;
+      segs[nseg++] = s1;
+      for (int sg = 0; sg + 1 < nseg; sg++) {
+        const double ta = asinh(segs[sg] / d), tb = asinh(segs[sg + 1] / d);
+        const double span = fabs(tb - ta) / tau_panel;
+        int npan = span < 64.0 ? (int)ceil(span) : 64;  // also maps NaN to 1 below
+        if (!(npan >= 1)) npan = 1;
+        double* sd = segd[ns];
+        sd[0] = fx; sd[1] = fy; sd[2] = ex; sd[3] = ey; sd[4] = d; sd[5] = ta; sd[6] = tb; sd[7] = npan;
+        segc[ns++] = cnt;
+        cnt += npan * nn;
       }
     }
+    segc[ns] = cnt;
+    for (int k = ns + 1; k < 7; k++) segc[k] = cnt;
+  }
+  __syncthreads();
+  const int total = segc[6];
+  for (int base = 0; base < total; base += WAVE) {
+    const int k = base + lane;
+    if (k < total) {
+      int sgi = 0;
+      while (sgi < 5 && k >= segc[sgi + 1]) sgi++;
+      const double* sd = segd[sgi];
+      const double fx = sd[0], fy = sd[1], ex = sd[2], ey = sd[3], d = sd[4], ta = sd[5], tb = sd[6];
+      const int npan = (int)sd[7];
+      const int loc = k - segc[sgi];
+      const int ipan = loc / nn, kk = loc - ipan * nn;
+      const double t0 = ta + (tb - ta) * ipan / npan, t1 = ta + (tb - ta) * (ipan + 1) / npan;
+      double* row = buf + lane * row_w(NP);
+      // kk < 256: a float reciprocal gives the exact quotient
+      const int it = (int)(((float)kk + 0.5f) * inv_nr), ir = kk - it * nr;
+      const double tau = t0 + (t1 - t0) * g_self_gl.x[it];
+      // sinh and cosh from one exponential (only d * sinh enters, so the absolute error
+      // of the difference near tau = 0 is what matters)
+      const double et = exp(tau), eti = 1.0 / et;
+      const double sh = 0.5 * (et - eti), ch = 0.5 * (et + eti);
+      const double wtau = g_self_gl.w[it] * (t1 - t0);
+      const double px = fx + d * sh * ex, py = fy + d * sh * ey;
+      const double rho = g_self_gl.x[ir];
+      const double xi0 = rho * px, xi1 = rho * py;
+      const double dv = xi1 / l22, du = (xi0 - l12 * dv) / l11;
+      double g[12];
+      koorn_geom_row<N>(u0 + du, v0 + dv, sc, g, row + 2);
+      const double jac = geom_finish(g);
+      const double w = g_self_gl.w[ir] * wtau * rho * d * d * ch / detl * jac;
+      double vr, vi;
+      kern_eval<HELM>(tg, g, kp, vr, vi);
+      row[0] = vr * w;
+      row[1] = vi * w;
+    }
+    const int nrows = total - base < WAVE ? total - base : WAVE;
+    acc += column_sum<NP, HELM>(buf, nrows);
   }
   store_weights<NP, HELM>(buf, acc, umat, wnear, iquad[jt.x] - 1);
 }
