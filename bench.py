@@ -150,6 +150,17 @@ def main():
     # one detailed run for the work counts (refinements per pair)
     nq.run(want_detail=True)
     st = nq.stats
+    # the same kernels alone: every kernel in order on one stream (in the timed steps the
+    # fixed-rule and self kernels run beside the adaptive one and share the GPU with it)
+    layer.set_side_stream(False)
+    iso = {"adaptive": [], "fixed": [], "self": []}
+    for _ in range(max(3, args.steps // 4)):
+        nq.run()
+        iso["adaptive"].append(nq.stats["ms_adap"])
+        iso["fixed"].append(nq.stats["ms_fixed"])
+        iso["self"].append(nq.stats["ms_self"])
+    layer.set_side_stream(True)
+    iso = {k: float(np.mean(v)) for k, v in iso.items()}
     fk = F_K_HELM if helm else F_K_LAP
     adap_fl, fixed_fl, nref = pair_flops(st, p, st["kq"], st["nf"], fk)
     ka = float(np.mean(ms_adap)) * 1e-3
@@ -180,9 +191,14 @@ def main():
                    "refinements_mean": float(nref.mean()) if nref.size else 0.0},
         "kernels_ms": {"prep": float(np.mean(ms_prep)), "fixed": kf * 1e3, "adaptive": ka * 1e3,
                        "self": kself * 1e3},
+        "kernels_note": "timed steps: fixed-rule and self kernels on a second stream beside the adaptive one; "
+                        "kernels_ms_alone: each kernel alone, in order on one stream",
+        "kernels_ms_alone": iso,
         "roofline": {"bound": "mfma", "pipe": "fp64 (VALU; MI355X FP64 vector peak = matrix peak)",
                      "kernel": "k_adaptive", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
+                     "alone": {"ms": iso["adaptive"], "achieved": fl / (iso["adaptive"] * 1e-3) / 1e12,
+                               "frac": fl / (iso["adaptive"] * 1e-3) / PEAK_FP64}},
         "setup_host_s": t_setup_host,
         "setup_gpu_near_s": t_setup_gpu,
         "setup_gpu_near_identical": setup_same,
@@ -401,12 +417,16 @@ def learned_mixed_solve(dev):
 
     plain()
     (x64, nit, rr64, ts64), tt64 = _timed(plain)
-    nq.run()
-    q64 = {k: nq.stats[k] for k in ("ms_tree", "ms_adap", "ms_fixed", "ms_self")}
     learned()
     (xl, outer, inner, rrl, tsl), ttl = _timed(learned)
+    # per-kernel times of each kernel alone (every kernel in order on one stream)
+    layer.set_side_stream(False)
+    nq.run()
+    q64 = {k: nq.stats[k] for k in ("ms_tree", "ms_adap", "ms_fixed", "ms_self")}
+    nq.run(selector=params)
     ql = {k: nq.stats[k] for k in ("ms_select", "ms_tree", "ms_shallow", "ms_adap", "ms_fixed", "ms_self",
                                    "nshallow0", "nshallow1", "nfallback")}
+    layer.set_side_stream(True)
     targ = np.array([[4.0, 0.0, 1.0], [0.0, 4.0, -1.0], [-3.5, 1.0, 2.0], [0.5, -3.8, 0.5], [0.0, 0.0, 3.0],
                      [2.0, 2.0, 2.5], [-2.0, 2.5, -2.0], [3.0, -3.0, 0.0], [0.0, 0.0, -3.5], [-4.0, -1.0, 0.0]])
     exact = problem.point_source(targ, x0, zk)

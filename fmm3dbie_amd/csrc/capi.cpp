@@ -1013,6 +1013,9 @@ int fmm3dbie_far_order_dev(double eps, int64_t npatches, int norder, const int64
 // Free the device work buffers the near-quadrature calls keep between calls.
 void fmm3dbie_release_workspace() { fmm_nearquad_release(); }
 
+// Concurrency of the near-quadrature kernels (see the header).
+void fmm3dbie_set_side_stream(int enable) { fmm_nearquad_set_side_stream(enable); }
+
 // ---------------------------------------------------------------- learned selector
 int fmm3dbie_mlp_nparam() { return MLP_NPARAM; }
 

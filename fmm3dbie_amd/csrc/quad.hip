@@ -1039,6 +1039,10 @@ struct WsBuf {
   size_t bytes = 0;
 };
 std::mutex g_quad_mutex;
+// per device: a second stream for the fixed-rule and self kernels, which run beside the
+// tree cache and the adaptive kernel (their pairs and weight slices are disjoint)
+std::map<int, hipStream_t> g_side;
+int g_use_side = 1;
 std::map<int, std::vector<WsBuf>> g_ws;
 
 template <typename T>
@@ -1207,7 +1211,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   kp.ar = A.alpha[0]; kp.ai = A.alpha[1];
   kp.br = A.beta[0]; kp.bi = A.beta[1];
 
-  hipEvent_t ev[10];
+  hipEvent_t ev[11];
   for (auto& e : ev) HCHK(hipEventCreate(&e));
   HCHK(hipEventRecord(ev[0], s));
   const int tb = 256;
@@ -1224,12 +1228,37 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   HCHK(hipStreamSynchronize(s));
   const int nadap = counts[0], nself = counts[1];
   HCHK(hipEventRecord(ev[1], s));
-  // fixed
+  // fixed-rule and self pairs on the side stream, after the classification
+  hipStream_t s2 = s;
+  if (g_use_side) {
+    int dev = 0;
+    HCHK(hipGetDevice(&dev));
+    auto it = g_side.find(dev);
+    if (it == g_side.end()) {
+      hipStream_t ns;
+      HCHK(hipStreamCreateWithFlags(&ns, hipStreamNonBlocking));
+      it = g_side.emplace(dev, ns).first;
+    }
+    s2 = it->second;
+  }
+  if (nself > 0 && !A.uvs_targ) {
+    fprintf(stderr, "fmm3dbie_amd: self pairs present but no uvs_targ\n");
+    return -3;
+  }
+  if (s2 != s) HCHK(hipStreamWaitEvent(s2, ev[1], 0));
   const size_t lds_fixed = sizeof(double) * (9 * NP + 7 * (size_t)nf);
-  k_fixed<N, HELM><<<npat, 128, lds_fixed, s>>>(nf, d_fuv, d_Pq, d_Bq, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
-                                                A.iquad, d_off, d_fixed, kp, A.wnear);
+  k_fixed<N, HELM><<<npat, 128, lds_fixed, s2>>>(nf, d_fuv, d_Pq, d_Bq, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
+                                                 A.iquad, d_off, d_fixed, kp, A.wnear);
   HCHK(hipGetLastError());
-  HCHK(hipEventRecord(ev[2], s));
+  HCHK(hipEventRecord(ev[2], s2));
+  if (nself > 0) {
+    k_self<N, HELM><<<nself, 64, 0, s2>>>(nself, d_self, A.col_ind, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
+                                          A.uvs_targ, A.iquad, d_umat, kp, A.self_n, A.self_n, A.self_tau_panel,
+                                          A.wnear);
+    HCHK(hipGetLastError());
+  }
+  HCHK(hipEventRecord(ev[4], s2));
+  HCHK(hipEventRecord(ev[10], s));
   const int ncn = NCACHE_TRI * kq;
   double *d_tuv = TB.tuv, *d_twq = TB.twq, *d_ptab = TB.ptab;
   double* d_cache = nadap > 0 ? wsalloc<double>(W_CACHE, (size_t)npat * CACHE_STRIDE * ncn) : nullptr;
@@ -1322,17 +1351,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     HCHK(hipEventRecord(ev[7], s));
   }
   HCHK(hipEventRecord(ev[3], s));
-  if (nself > 0) {
-    if (!A.uvs_targ) {
-      fprintf(stderr, "fmm3dbie_amd: self pairs present but no uvs_targ\n");
-      return -3;
-    }
-    k_self<N, HELM><<<nself, 64, 0, s>>>(nself, d_self, A.col_ind, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
-                                         A.uvs_targ, A.iquad, d_umat, kp, A.self_n, A.self_n, A.self_tau_panel,
-                                         A.wnear);
-    HCHK(hipGetLastError());
-  }
-  HCHK(hipEventRecord(ev[4], s));
+  if (s2 != s) HCHK(hipStreamWaitEvent(s, ev[4], 0));  // the side stream's fixed-rule and self kernels
   int counts8[8];
   HCHK(hipMemcpyAsync(counts8, d_counts, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
   HCHK(hipStreamSynchronize(s));
@@ -1348,7 +1367,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     st->kq = kq;
     hipEventElapsedTime(&ms, ev[0], ev[1]); st->ms_prep = ms;
     hipEventElapsedTime(&ms, ev[1], ev[2]); st->ms_fixed = ms;
-    hipEventElapsedTime(&ms, ev[2], ev[5]); st->ms_tables = ms;
+    hipEventElapsedTime(&ms, ev[10], ev[5]); st->ms_tables = ms;
     hipEventElapsedTime(&ms, ev[5], ev[6]); st->ms_select = ms;
     hipEventElapsedTime(&ms, ev[6], ev[8]); st->ms_tree = ms;
     hipEventElapsedTime(&ms, ev[8], ev[7]); st->ms_shallow = ms;
@@ -1356,7 +1375,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     st->nshallow0 = counts8[5];
     st->nshallow1 = counts8[6];
     st->nfallback = counts8[4];
-    hipEventElapsedTime(&ms, ev[3], ev[4]); st->ms_self = ms;
+    hipEventElapsedTime(&ms, ev[2], ev[4]); st->ms_self = ms;
     st->nref_total = 0;
     if (d_nref && A.nref_host) {
       hipMemcpy(A.nref_host, d_nref, sizeof(int) * A.nnz, hipMemcpyDeviceToHost);
@@ -1402,8 +1421,22 @@ int fmm_nearquad_device(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s)
 
 int fmm_koorn_constants() { return ensure_constants(); }
 
+void fmm_nearquad_set_side_stream(int enable) {
+  std::lock_guard<std::mutex> lock(g_quad_mutex);
+  g_use_side = enable ? 1 : 0;
+}
+
 void fmm_nearquad_release() {
   std::lock_guard<std::mutex> lock(g_quad_mutex);
+  for (auto& kv : g_side) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(kv.first);
+    (void)hipStreamSynchronize(kv.second);
+    (void)hipStreamDestroy(kv.second);
+    (void)hipSetDevice(cur);
+  }
+  g_side.clear();
   for (auto& kv : g_ws)
     for (auto& b : kv.second)
       if (b.p) (void)hipFree(b.p);

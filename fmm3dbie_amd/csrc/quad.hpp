@@ -52,4 +52,7 @@ struct NearQuadStats {
 // (grow-only) until fmm_nearquad_release().
 int fmm_nearquad_device(const NearQuadArgs& a, NearQuadStats* st, hipStream_t s);
 void fmm_nearquad_release();
+// 1 (default): the fixed-rule and self kernels run on a second stream beside the tree
+// cache and the adaptive kernels; 0: everything on the caller's stream, in order
+void fmm_nearquad_set_side_stream(int enable);
 int fmm_koorn_constants();

@@ -273,6 +273,13 @@ def kpar6(helm, pars):
 MAX_QUAD_ORDER = 9
 
 
+def set_side_stream(enable=True):
+    """Near-quadrature concurrency: True (default) runs the fixed-rule and self kernels on
+    a second stream beside the adaptive kernels; False keeps every kernel in order on the
+    caller's stream, so that per-kernel times are those of the kernel alone."""
+    lib().fmm3dbie_set_side_stream(int(bool(enable)))
+
+
 class NearQuadDevice:
     """Near quadrature with device-resident inputs (the benchmarked path).
 

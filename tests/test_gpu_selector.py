@@ -95,6 +95,7 @@ def test_trained_selector_parity_and_gain(cuda):
     params, info = S.train(X, y, n_train=int(0.7 * len(y)), epochs=30, seed=3, device=cuda)
     assert info["acc_heldout"] > 0.6, info
     plain, learned, parts = [], [], []
+    layer.set_side_stream(False)  # per-kernel times of the kernels alone
     for _ in range(3):
         nq.run()
         plain.append(nq.stats["ms_tree"] + nq.stats["ms_adap"])
@@ -102,6 +103,7 @@ def test_trained_selector_parity_and_gain(cuda):
         st = nq.stats
         learned.append(st["ms_select"] + st["ms_tree"] + st["ms_shallow"] + st["ms_adap"])
         parts.append((st["ms_select"], st["ms_tree"], st["ms_shallow"], st["ms_adap"]))
+    layer.set_side_stream(True)
     print("selector parts (select, tree, cached-tree kernels, adaptive) ms:", parts[int(np.argmin(learned))])
     w = nq.run(want_detail=True, selector=params).cpu().numpy()
     assert np.array_equal(nq.stats["nref"][idx], nref0[idx])
