@@ -1211,9 +1211,37 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   kp.ar = A.alpha[0]; kp.ai = A.alpha[1];
   kp.br = A.beta[0]; kp.bi = A.beta[1];
 
-  hipEvent_t ev[11];
+  hipEvent_t ev[12];
   for (auto& e : ev) HCHK(hipEventCreate(&e));
   HCHK(hipEventRecord(ev[0], s));
+  // side stream: the tree cache (on-surface targets, where adaptive pairs are all but
+  // certain: it then runs beside the classification and its host read-back), then the
+  // fixed-rule and self kernels
+  hipStream_t s2 = s;
+  if (g_use_side) {
+    int dev = 0;
+    HCHK(hipGetDevice(&dev));
+    auto it = g_side.find(dev);
+    if (it == g_side.end()) {
+      hipStream_t ns;
+      HCHK(hipStreamCreateWithFlags(&ns, hipStreamNonBlocking));
+      it = g_side.emplace(dev, ns).first;
+    }
+    s2 = it->second;
+  }
+  const int ncn = NCACHE_TRI * kq;
+  double *d_tuv = TB.tuv, *d_twq = TB.twq, *d_ptab = TB.ptab;
+  const bool early_tree = s2 != s && A.ipatch_id != nullptr && npat > 0;
+  double* d_cache = nullptr;
+  if (early_tree) {
+    d_cache = wsalloc<double>(W_CACHE, (size_t)npat * CACHE_STRIDE * ncn);
+    if (!d_cache) return -2;
+    HCHK(hipStreamWaitEvent(s2, ev[0], 0));
+    HCHK(hipEventRecord(ev[9], s2));
+    k_tree_cache<N><<<npat, 256, 0, s2>>>(npat, kq, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs, d_cache);
+    HCHK(hipGetLastError());
+    HCHK(hipEventRecord(ev[8], s2));
+  }
   const int tb = 256;
   k_patch_prep<N><<<(npat + tb - 1) / tb, tb, 0, s>>>(npat, A.ixyzs, A.srccoefs, A.srcvals, d_rvw, eps_adap,
                                                       d_cms, d_rads, d_epsp, d_area);
@@ -1229,18 +1257,6 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   const int nadap = counts[0], nself = counts[1];
   HCHK(hipEventRecord(ev[1], s));
   // fixed-rule and self pairs on the side stream, after the classification
-  hipStream_t s2 = s;
-  if (g_use_side) {
-    int dev = 0;
-    HCHK(hipGetDevice(&dev));
-    auto it = g_side.find(dev);
-    if (it == g_side.end()) {
-      hipStream_t ns;
-      HCHK(hipStreamCreateWithFlags(&ns, hipStreamNonBlocking));
-      it = g_side.emplace(dev, ns).first;
-    }
-    s2 = it->second;
-  }
   if (nself > 0 && !A.uvs_targ) {
     fprintf(stderr, "fmm3dbie_amd: self pairs present but no uvs_targ\n");
     return -3;
@@ -1259,10 +1275,10 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   }
   HCHK(hipEventRecord(ev[4], s2));
   HCHK(hipEventRecord(ev[10], s));
-  const int ncn = NCACHE_TRI * kq;
-  double *d_tuv = TB.tuv, *d_twq = TB.twq, *d_ptab = TB.ptab;
-  double* d_cache = nadap > 0 ? wsalloc<double>(W_CACHE, (size_t)npat * CACHE_STRIDE * ncn) : nullptr;
-  if (nadap > 0 && !d_cache) return -2;
+  if (!early_tree && nadap > 0) {
+    d_cache = wsalloc<double>(W_CACHE, (size_t)npat * CACHE_STRIDE * ncn);
+    if (!d_cache) return -2;
+  }
   // adaptive
   const int nwaves = A.adaptive_waves > 0 ? A.adaptive_waves : 4096;
   constexpr int ES = 6 + 2 * NP;
@@ -1324,8 +1340,14 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   }
   HCHK(hipEventRecord(ev[6], s));
   if (nadap > 0) {
-    k_tree_cache<N><<<npat, 256, 0, s>>>(npat, kq, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs, d_cache);
-    HCHK(hipEventRecord(ev[8], s));
+    if (early_tree) {
+      HCHK(hipStreamWaitEvent(s, ev[8], 0));
+    } else {
+      HCHK(hipEventRecord(ev[9], s));
+      k_tree_cache<N><<<npat, 256, 0, s>>>(npat, kq, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs, d_cache);
+      HCHK(hipEventRecord(ev[8], s));
+    }
+    HCHK(hipEventRecord(ev[11], s));
     if (learned) {
       // grid = 8 x chunk workgroups, chunk = 16-pair blocks per XCD
       const int c0 = ((nb[0] + 15) / 16 + 7) / 8, c1 = ((nb[1] + 15) / 16 + 7) / 8;
@@ -1347,7 +1369,11 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
                                          d_ptab);
     HCHK(hipGetLastError());
   } else {
-    HCHK(hipEventRecord(ev[8], s));
+    if (!early_tree) {
+      HCHK(hipEventRecord(ev[9], s));
+      HCHK(hipEventRecord(ev[8], s));
+    }
+    HCHK(hipEventRecord(ev[11], s));
     HCHK(hipEventRecord(ev[7], s));
   }
   HCHK(hipEventRecord(ev[3], s));
@@ -1369,8 +1395,8 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     hipEventElapsedTime(&ms, ev[1], ev[2]); st->ms_fixed = ms;
     hipEventElapsedTime(&ms, ev[10], ev[5]); st->ms_tables = ms;
     hipEventElapsedTime(&ms, ev[5], ev[6]); st->ms_select = ms;
-    hipEventElapsedTime(&ms, ev[6], ev[8]); st->ms_tree = ms;
-    hipEventElapsedTime(&ms, ev[8], ev[7]); st->ms_shallow = ms;
+    hipEventElapsedTime(&ms, ev[9], ev[8]); st->ms_tree = ms;
+    hipEventElapsedTime(&ms, ev[11], ev[7]); st->ms_shallow = ms;
     hipEventElapsedTime(&ms, ev[7], ev[3]); st->ms_adap = ms;
     st->nshallow0 = counts8[5];
     st->nshallow1 = counts8[6];

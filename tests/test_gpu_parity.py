@@ -172,3 +172,19 @@ def test_solver_guru_dropin(cuda):
 def test_smoke_entry(cuda):
     import __graft_entry__
     __graft_entry__.smoke()
+
+
+def test_side_stream_gives_the_same_weights(cuda):
+    """The fixed-rule and self kernels (and, for on-surface targets, the tree cache) on the
+    library's second stream give exactly the weights of the in-order run on one stream."""
+    from fmm3dbie_amd import layer
+    g, helm, pars, P = _case("helm", 4, 2)
+    nq = layer.NearQuadDevice(g, helm, pars, 1e-6, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], device=cuda)
+    try:
+        layer.set_side_stream(False)
+        w0 = nq.run().cpu().numpy()
+        layer.set_side_stream(True)
+        for _ in range(3):
+            assert np.array_equal(nq.run().cpu().numpy(), w0)
+    finally:
+        layer.set_side_stream(True)
