@@ -13,16 +13,19 @@ def _rel(a, b):
 
 
 def _case(kind, norder, na, eps=1e-6):
+    """helm: zk = 1.1; helmc: complex zk = 1.1 + 0.3i (decaying kernel); lap: Laplace."""
     from fmm3dbie_amd import geometry as G, problem
-    helm = kind == "helm"
+    helm = kind in ("helm", "helmc")
     g = G.get_sphere_npat(1.0, na, (0, 0, 0), norder)
-    pars = np.array([1.1, 1.1j, 1.0]) if helm else np.array([1.0, 1.0])
+    zk = 1.1 + 0.3j if kind == "helmc" else 1.1
+    pars = np.array([zk, 1j * zk, 1.0]) if helm else np.array([1.0, 1.0])
     P = problem.setup(g, eps, helm, pars)
     return g, helm, pars, P
 
 
 # orders 8 and 9: 2 * (p + 1)(p + 2) / 2 moment components exceed the 64 lanes of a wavefront
-CASES = [("helm", 4, 2), ("lap", 3, 2), ("helm", 6, 2), ("helm", 3, 3), ("helm", 8, 2), ("lap", 9, 1)]
+CASES = [("helm", 4, 2), ("lap", 3, 2), ("helm", 6, 2), ("helm", 3, 3), ("helm", 8, 2), ("lap", 9, 1),
+         ("helmc", 4, 2)]
 
 
 @pytest.mark.parametrize("kind,norder,na", CASES)
@@ -80,7 +83,7 @@ def test_eval_offsurface_targets_matches_oracle(cuda):
     assert _rel(w, ref) < 1e-12
 
 
-@pytest.mark.parametrize("kind,norder,na", [("helm", 4, 2), ("lap", 3, 2)])
+@pytest.mark.parametrize("kind,norder,na", [("helm", 4, 2), ("lap", 3, 2), ("helmc", 4, 2)])
 def test_lpcomp_matches_oracle(cuda, kind, norder, na):
     import torch
     from fmm3dbie_amd import layer
