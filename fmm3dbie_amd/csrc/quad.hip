@@ -42,6 +42,11 @@ namespace {
 constexpr int WAVE = 64;
 constexpr int STACK_CAP = 72;
 constexpr int NLMAX = 20;
+// The second stream is used on surfaces of up to SIDE_MAX_PATCHES patches: measured 7-8%
+// faster on cube spheres of 768-1728 patches of orders 4-6, equal on a 1536-patch order-6
+// torus, and 6-8% slower on a 12288-patch order-6 surface, where the adaptive kernel alone
+// keeps the GPU busy and the kernels beside it only take its slots.
+constexpr int64_t SIDE_MAX_PATCHES = 4096;
 
 struct GLRule {
   double x[16];
@@ -1211,14 +1216,14 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   kp.ar = A.alpha[0]; kp.ai = A.alpha[1];
   kp.br = A.beta[0]; kp.bi = A.beta[1];
 
-  hipEvent_t ev[12];
+  hipEvent_t ev[14];
   for (auto& e : ev) HCHK(hipEventCreate(&e));
   HCHK(hipEventRecord(ev[0], s));
   // side stream: the tree cache (on-surface targets, where adaptive pairs are all but
   // certain: it then runs beside the classification and its host read-back), then the
   // fixed-rule and self kernels
   hipStream_t s2 = s;
-  if (g_use_side) {
+  if (g_use_side && npat <= SIDE_MAX_PATCHES) {
     int dev = 0;
     HCHK(hipGetDevice(&dev));
     auto it = g_side.find(dev);
@@ -1267,12 +1272,19 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
                                                  A.iquad, d_off, d_fixed, kp, A.wnear);
   HCHK(hipGetLastError());
   HCHK(hipEventRecord(ev[2], s2));
-  if (nself > 0) {
-    k_self<N, HELM><<<nself, 64, 0, s2>>>(nself, d_self, A.col_ind, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
-                                          A.uvs_targ, A.iquad, d_umat, kp, A.self_n, A.self_n, A.self_tau_panel,
-                                          A.wnear);
-    HCHK(hipGetLastError());
-  }
+  const bool self_side = s2 != s;
+  auto launch_self = [&](hipStream_t ss) -> int {
+    HCHK(hipEventRecord(ev[12], ss));
+    if (nself > 0) {
+      k_self<N, HELM><<<nself, 64, 0, ss>>>(nself, d_self, A.col_ind, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
+                                            A.uvs_targ, A.iquad, d_umat, kp, A.self_n, A.self_n,
+                                            A.self_tau_panel, A.wnear);
+      HCHK(hipGetLastError());
+    }
+    HCHK(hipEventRecord(ev[13], ss));
+    return 0;
+  };
+  if (self_side && launch_self(s2) != 0) return -1;
   HCHK(hipEventRecord(ev[4], s2));
   HCHK(hipEventRecord(ev[10], s));
   if (!early_tree && nadap > 0) {
@@ -1377,7 +1389,8 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     HCHK(hipEventRecord(ev[7], s));
   }
   HCHK(hipEventRecord(ev[3], s));
-  if (s2 != s) HCHK(hipStreamWaitEvent(s, ev[4], 0));  // the side stream's fixed-rule and self kernels
+  if (s2 != s) HCHK(hipStreamWaitEvent(s, ev[4], 0));  // the side stream's kernels
+  if (!self_side && launch_self(s) != 0) return -1;
   int counts8[8];
   HCHK(hipMemcpyAsync(counts8, d_counts, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
   HCHK(hipStreamSynchronize(s));
@@ -1401,7 +1414,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     st->nshallow0 = counts8[5];
     st->nshallow1 = counts8[6];
     st->nfallback = counts8[4];
-    hipEventElapsedTime(&ms, ev[2], ev[4]); st->ms_self = ms;
+    hipEventElapsedTime(&ms, ev[12], ev[13]); st->ms_self = ms;
     st->nref_total = 0;
     if (d_nref && A.nref_host) {
       hipMemcpy(A.nref_host, d_nref, sizeof(int) * A.nnz, hipMemcpyDeviceToHost);

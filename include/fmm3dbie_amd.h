@@ -255,10 +255,11 @@ int fmm3dbie_far_order_dev(double eps, int64_t npatches, int norder, const int64
  * No reference counterpart. */
 void fmm3dbie_release_workspace(void);
 
-/* Near-quadrature concurrency (no reference counterpart): with enable = 1 (default) the
- * fixed-rule and self-quadrature kernels run on a library-owned second stream beside the
- * adaptive kernels (disjoint pairs and weights; the call returns after both streams);
- * 0 runs every kernel in order on the caller's stream (per-kernel timings undisturbed). */
+/* Near-quadrature concurrency (no reference counterpart): with enable = 1 (default) and
+ * surfaces of up to 4096 patches, the tree cache, fixed-rule and self-quadrature kernels
+ * run on a library-owned second stream beside the adaptive kernels (disjoint pairs and
+ * weights; the call returns after both streams); 0 runs every kernel in order on the
+ * caller's stream (per-kernel timings undisturbed). */
 void fmm3dbie_set_side_stream(int enable);
 
 /* ---- learned refinement-depth selector (no reference counterpart) --------------------
