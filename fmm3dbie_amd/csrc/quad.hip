@@ -1216,8 +1216,16 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   kp.ar = A.alpha[0]; kp.ai = A.alpha[1];
   kp.br = A.beta[0]; kp.bi = A.beta[1];
 
-  hipEvent_t ev[14];
-  for (auto& e : ev) HCHK(hipEventCreate(&e));
+  // timing events, destroyed on every return path
+  struct Events {
+    hipEvent_t e[14] = {};
+    ~Events() {
+      for (auto& x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } evs;
+  hipEvent_t* ev = evs.e;
+  for (auto& e : evs.e) HCHK(hipEventCreate(&e));
   HCHK(hipEventRecord(ev[0], s));
   // side stream: the tree cache (on-surface targets, where adaptive pairs are all but
   // certain: it then runs beside the classification and its host read-back), then the
@@ -1395,33 +1403,34 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   HCHK(hipMemcpyAsync(counts8, d_counts, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
   HCHK(hipStreamSynchronize(s));
   for (int i = 0; i < 4; i++) counts[i] = counts8[i];
-  if (d_pfull) hipMemcpy(A.pred_host, d_pfull, sizeof(int) * A.nnz, hipMemcpyDeviceToHost);
+  if (d_pfull) HCHK(hipMemcpy(A.pred_host, d_pfull, sizeof(int) * A.nnz, hipMemcpyDeviceToHost));
   if (st) {
-    float ms;
     st->nself = nself;
     st->nadap = nadap;
     st->nfixed = (int)(A.nnz - nself - nadap);
     st->overflow = counts[2];
     st->nf = nf;
     st->kq = kq;
-    hipEventElapsedTime(&ms, ev[0], ev[1]); st->ms_prep = ms;
-    hipEventElapsedTime(&ms, ev[1], ev[2]); st->ms_fixed = ms;
-    hipEventElapsedTime(&ms, ev[10], ev[5]); st->ms_tables = ms;
-    hipEventElapsedTime(&ms, ev[5], ev[6]); st->ms_select = ms;
-    hipEventElapsedTime(&ms, ev[9], ev[8]); st->ms_tree = ms;
-    hipEventElapsedTime(&ms, ev[11], ev[7]); st->ms_shallow = ms;
-    hipEventElapsedTime(&ms, ev[7], ev[3]); st->ms_adap = ms;
+    auto elapsed = [&](int a, int b) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
+      return (double)ms;
+    };
+    st->ms_prep = elapsed(0, 1);
+    st->ms_fixed = elapsed(1, 2);
+    st->ms_tables = elapsed(10, 5);
+    st->ms_select = elapsed(5, 6);
+    st->ms_tree = elapsed(9, 8);
+    st->ms_shallow = elapsed(11, 7);
+    st->ms_adap = elapsed(7, 3);
+    st->ms_self = elapsed(12, 13);
     st->nshallow0 = counts8[5];
     st->nshallow1 = counts8[6];
     st->nfallback = counts8[4];
-    hipEventElapsedTime(&ms, ev[12], ev[13]); st->ms_self = ms;
     st->nref_total = 0;
-    if (d_nref && A.nref_host) {
-      hipMemcpy(A.nref_host, d_nref, sizeof(int) * A.nnz, hipMemcpyDeviceToHost);
-    }
-    if (A.class_host) hipMemcpy(A.class_host, d_cls, sizeof(int) * A.nnz, hipMemcpyDeviceToHost);
+    if (d_nref && A.nref_host) HCHK(hipMemcpy(A.nref_host, d_nref, sizeof(int) * A.nnz, hipMemcpyDeviceToHost));
+    if (A.class_host) HCHK(hipMemcpy(A.class_host, d_cls, sizeof(int) * A.nnz, hipMemcpyDeviceToHost));
   }
-  for (auto& e : ev) hipEventDestroy(e);
   (void)d_umat;
   return 0;
 }
