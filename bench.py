@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-plane-wave batched solve")
     ap.add_argument("--batched-sequential", type=int, default=8,
                     help="FP64 single-RHS solves timed for the batched comparison")
+    ap.add_argument("--apply-only", action="store_true",
+                    help="only the near quadrature once, then 5 FP64 (add/subtract) and 5 FP32 (masked) applies "
+                         "of the workload (for the apply's rocprofv3 counters)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device read + 1 GiB write (torch.neg) for HBM counter scaling")
     args = ap.parse_args()
@@ -126,6 +129,9 @@ def main():
     N = len(g["srcvals"])
     p = int(np.diff(g["ixyzs"])[0])
     nq = layer.NearQuadDevice(g, helm, pars, eps, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], device=dev)
+    if args.apply_only:
+        print(json.dumps(apply_only(g, helm, pars, P, nq, dev)))
+        return
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         nq.run()
@@ -310,6 +316,32 @@ def _timed(fn):
     r = fn()
     torch.cuda.synchronize()
     return r, time.perf_counter() - t0
+
+
+def apply_only(g, helm, pars, P, nq, dev):
+    """5 FP64 applies in the reference's add/subtract algebra and 5 FP32 masked applies of
+    a random density, with the per-phase HIP-event times (ms): oversampling, far sum,
+    split sum + CSR correction, subtraction."""
+    import torch
+    from fmm3dbie_amd import layer
+    w = nq.run()
+    plan = layer.ApplyPlan(g, helm, pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"], P["ixyzso"],
+                           P["srcover"], P["whtsover"])
+    npts, nptso, _, _ = plan.info()
+    rng = np.random.default_rng(0)
+    sig = torch.as_tensor(rng.standard_normal(npts) + 1j * rng.standard_normal(npts), device=dev)
+    out = {"workload_points": npts, "oversampled_sources": nptso, "nnz": P["nnz"], "nquad": P["nquad"]}
+    for name, prec, mode, s in (("fp64_addsub", 64, 0, sig), ("fp32_masked", 32, 1, sig.to(torch.complex64))):
+        tm = np.zeros(4)
+        acc = []
+        plan.apply(s, prec, mode)
+        for _ in range(5):
+            plan.apply(s, prec, mode, times=tm)
+            acc.append(tm.copy())
+        torch.cuda.synchronize()
+        out[name + "_ms"] = dict(zip(("oversample", "far", "csr", "subtract"), np.mean(acc, axis=0).tolist()))
+    plan.close()
+    return out
 
 
 def large_apply(g, pars, P, wnear, dev):
