@@ -66,6 +66,26 @@ __global__ void k_oversample(int64_t nptso, const int* __restrict__ opatch, cons
   }
 }
 
+// 16-byte LDS word of T values (4 floats or 2 doubles)
+template <typename T>
+struct Word16;
+template <>
+struct Word16<float> {
+  using type = float4;
+  static __device__ __forceinline__ float4 pack(const float* v) { return make_float4(v[0], v[1], v[2], v[3]); }
+  static __device__ __forceinline__ void unpack(const float4 w, float* v) {
+    v[0] = w.x; v[1] = w.y; v[2] = w.z; v[3] = w.w;
+  }
+};
+template <>
+struct Word16<double> {
+  using type = double2;
+  static __device__ __forceinline__ double2 pack(const double* v) { return make_double2(v[0], v[1]); }
+  static __device__ __forceinline__ void unpack(const double2 w, double* v) {
+    v[0] = w.x; v[1] = w.y;
+  }
+};
+
 // e^{ikr}/r and the dipole factor; T = double or float.  GEN: complex wavenumber, or (FP32)
 // phases beyond the hardware sine's range, reduced to one revolution first; otherwise the
 // FP32 phase k r / (2 pi) = r * krev goes to the hardware sine and cosine directly
@@ -125,8 +145,14 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
                                                 const int64_t* __restrict__ row_ptr,
                                                 const int* __restrict__ near_sorted, T kr, T krev, T ki, T thresh2,
                                                 int64_t chunk, V2<T>* __restrict__ part, int64_t ldpart) {
-  __shared__ T lx[TPB], ly[TPB], lz[TPB], lnx[TPB], lny[TPB], lnz[TPB];
-  __shared__ V2<T> lq[NRHS][TPB], lm[NRHS][TPB];
+  // One LDS record per source of the tile: x, y, z, nx, ny, nz, then (q, m) of each
+  // right-hand side, padded to whole 16-byte words, so that a pair's operands are a few
+  // 16-byte reads at immediate offsets from one address (all lanes read the same record).
+  using W16 = typename Word16<T>::type;
+  constexpr int PER = 16 / (int)sizeof(T);            // values per 16-byte word
+  constexpr int NV = 6 + 4 * NRHS;                    // values per source
+  constexpr int REC = (NV + PER - 1) / PER;           // 16-byte words per source
+  __shared__ W16 tile[TPB * REC];
   // blockIdx.z selects a group of NRHS right-hand sides: the geometry and e^{ikr} of a
   // pair are computed once per group
   const int64_t rg = blockIdx.z;
@@ -165,13 +191,18 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
   for (int64_t base = jbeg; base < jend; base += TPB) {
     const int64_t j = base + threadIdx.x;
     if (j < jend) {
-      lx[threadIdx.x] = sx[j]; ly[threadIdx.x] = sy[j]; lz[threadIdx.x] = sz[j];
-      lnx[threadIdx.x] = snx[j]; lny[threadIdx.x] = sny[j]; lnz[threadIdx.x] = snz[j];
+      T v[REC * PER];
+      v[0] = sx[j]; v[1] = sy[j]; v[2] = sz[j];
+      v[3] = snx[j]; v[4] = sny[j]; v[5] = snz[j];
 #pragma unroll
       for (int r = 0; r < NRHS; r++) {
-        lq[r][threadIdx.x] = qstr[r * ldstr + j];
-        lm[r][threadIdx.x] = mstr[r * ldstr + j];
+        const V2<T> q = qstr[r * ldstr + j], m = mstr[r * ldstr + j];
+        v[6 + 4 * r] = q.x; v[7 + 4 * r] = q.y; v[8 + 4 * r] = m.x; v[9 + 4 * r] = m.y;
       }
+#pragma unroll
+      for (int e = NV; e < REC * PER; e++) v[e] = (T)0;
+#pragma unroll
+      for (int w = 0; w < REC; w++) tile[threadIdx.x * REC + w] = Word16<T>::pack(v + w * PER);
     }
     __syncthreads();
     const int cnt = jend - base < TPB ? (int)(jend - base) : TPB;
@@ -179,9 +210,11 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
     for (int k = 0; k < cnt; k++) {
       const int64_t jk = base + k;
       if (MASK && jk >= eb) advance(jk);
+      T v[REC * PER];
+#pragma unroll
+      for (int w = 0; w < REC; w++) Word16<T>::unpack(tile[k * REC + w], v + w * PER);
       T er, ei, tdr, tdi, r2;
-      pair_kernel<T, GEN>(x - lx[k], y - ly[k], z - lz[k], lnx[k], lny[k], lnz[k], kr, krev, ki, er, ei, tdr, tdi,
-                          r2);
+      pair_kernel<T, GEN>(x - v[0], y - v[1], z - v[2], v[3], v[4], v[5], kr, krev, ki, er, ei, tdr, tdi, r2);
       // with the mask, sources within thresh of the target belong to its near patches
       const bool keep = MASK ? jk < ea : r2 > thresh2;
       constexpr bool SEL_FACTORS = NRHS > 2;  // zero the pair's factors once, not each sum
@@ -193,7 +226,7 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
       }
 #pragma unroll
       for (int r = 0; r < NRHS; r++) {
-        const V2<T> q = lq[r][k], m = lm[r][k];
+        const V2<T> q = {v[6 + 4 * r], v[7 + 4 * r]}, m = {v[8 + 4 * r], v[9 + 4 * r]};
         // inner = q + m*(tdr + i tdi)
         const T ir = q.x + m.x * tdr - m.y * tdi;
         const T ii = q.y + m.x * tdi + m.y * tdr;
