@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <type_traits>
 #include <vector>
 
 #include "apply.hpp"
@@ -206,35 +207,53 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
     }
     __syncthreads();
     const int cnt = jend - base < TPB ? (int)(jend - base) : TPB;
+    // FULL: no source of the tile belongs to a near patch of any target of the wavefront
+    // (the next near interval of every lane starts at or after the tile's end), so every
+    // pair is summed and the per-pair test and selects drop out
+    auto pairs = [&](auto full_tag) {
+      constexpr bool FULL = decltype(full_tag)::value;
 #pragma unroll 2
-    for (int k = 0; k < cnt; k++) {
-      const int64_t jk = base + k;
-      if (MASK && jk >= eb) advance(jk);
-      T v[REC * PER];
+      for (int k = 0; k < cnt; k++) {
+        const int64_t jk = base + k;
+        if (MASK && !FULL && jk >= eb) advance(jk);
+        T v[REC * PER];
 #pragma unroll
-      for (int w = 0; w < REC; w++) Word16<T>::unpack(tile[k * REC + w], v + w * PER);
-      T er, ei, tdr, tdi, r2;
-      pair_kernel<T, GEN>(x - v[0], y - v[1], z - v[2], v[3], v[4], v[5], kr, krev, ki, er, ei, tdr, tdi, r2);
-      // with the mask, sources within thresh of the target belong to its near patches
-      const bool keep = MASK ? jk < ea : r2 > thresh2;
-      constexpr bool SEL_FACTORS = NRHS > 2;  // zero the pair's factors once, not each sum
-      if (SEL_FACTORS) {
-        er = keep ? er : (T)0;
-        ei = keep ? ei : (T)0;
-        tdr = keep ? tdr : (T)0;
-        tdi = keep ? tdi : (T)0;
-      }
+        for (int w = 0; w < REC; w++) Word16<T>::unpack(tile[k * REC + w], v + w * PER);
+        T er, ei, tdr, tdi, r2;
+        pair_kernel<T, GEN>(x - v[0], y - v[1], z - v[2], v[3], v[4], v[5], kr, krev, ki, er, ei, tdr, tdi, r2);
+        // with the mask, sources within thresh of the target belong to its near patches
+        const bool keep = FULL || (MASK ? jk < ea : r2 > thresh2);
+        constexpr bool SEL_FACTORS = NRHS > 2;  // zero the pair's factors once, not each sum
+        if (SEL_FACTORS) {
+          er = keep ? er : (T)0;
+          ei = keep ? ei : (T)0;
+          tdr = keep ? tdr : (T)0;
+          tdi = keep ? tdi : (T)0;
+        }
 #pragma unroll
-      for (int r = 0; r < NRHS; r++) {
-        const V2<T> q = {v[6 + 4 * r], v[7 + 4 * r]}, m = {v[8 + 4 * r], v[9 + 4 * r]};
-        // inner = q + m*(tdr + i tdi)
-        const T ir = q.x + m.x * tdr - m.y * tdi;
-        const T ii = q.y + m.x * tdi + m.y * tdr;
-        const T cr = ir * er - ii * ei, ci = ir * ei + ii * er;
-        accr[r] += SEL_FACTORS || keep ? cr : (T)0;
-        acci[r] += SEL_FACTORS || keep ? ci : (T)0;
+        for (int r = 0; r < NRHS; r++) {
+          const V2<T> q = {v[6 + 4 * r], v[7 + 4 * r]}, m = {v[8 + 4 * r], v[9 + 4 * r]};
+          // inner = q + m*(tdr + i tdi)
+          const T ir = q.x + m.x * tdr - m.y * tdi;
+          const T ii = q.y + m.x * tdi + m.y * tdr;
+          const T cr = ir * er - ii * ei, ci = ir * ei + ii * er;
+          accr[r] += SEL_FACTORS || keep ? cr : (T)0;
+          acci[r] += SEL_FACTORS || keep ? ci : (T)0;
+        }
       }
+    };
+    // (FP64 with several right-hand sides keeps the one loop: the second copy would cost
+    // it occupancy)
+    constexpr bool TRY_FULL = MASK && (sizeof(T) == 4 || NRHS == 1);
+    bool full = false;
+    if (TRY_FULL) {
+      if (base >= eb) advance(base);  // the state the first pair of the tile would reach
+      full = __all(ea >= base + cnt) != 0;  // wave-uniform
     }
+    if (full)
+      pairs(std::true_type{});
+    else
+      pairs(std::false_type{});
     __syncthreads();
   }
   if (act) {
