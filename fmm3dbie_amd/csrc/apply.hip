@@ -130,6 +130,35 @@ __device__ __forceinline__ void pair_kernel(T dx, T dy, T dz, T nx, T ny, T nz, 
   tdi = -kr * u;
 }
 
+// Far-sum pair factors with 1/r left to the caller: (c, s) = e^{i kr r} (times e^{-ki r}
+// when GEN), the dipole factor t (1 - i k r) = (tdr, tdi) with t = d / r^2, from r^2, d =
+// (x - y).n and rinv = 1/r.  A pair to be skipped passes rinv = 0: r is then 0 and every
+// factor stays finite, so its contribution (factors x rinv) is an exact zero.
+template <typename T, bool GEN>
+__device__ __forceinline__ void pair_factors(T r2, T rinv, T d, T kr, T krev, T ki, T& c, T& s, T& tdr, T& tdi) {
+  const T r = r2 * rinv;
+  if constexpr (sizeof(T) == 4) {
+    T rev = r * krev;
+    if constexpr (GEN) rev -= rintf(rev);
+    s = __builtin_amdgcn_sinf(rev);
+    c = __builtin_amdgcn_cosf(rev);
+  } else {
+    (void)krev;
+    sincos(kr * r, &s, &c);
+  }
+  if constexpr (GEN) {
+    T mag;
+    if constexpr (sizeof(T) == 4) mag = __expf(-ki * r);
+    else mag = exp(-ki * r);
+    c *= mag;
+    s *= mag;
+  }
+  const T u = d * rinv;
+  const T t = u * rinv;
+  tdr = GEN ? t * ((T)1 + ki * r) : t;
+  tdi = -kr * u;
+}
+
 // Tiled far sum.  pot[i] (+)= sum_j [q_j + m_j * t_ij (1-ikr)] e^{ikr}/r / (4 pi)
 // MASK: skip sources whose patch is in the target's sorted near list.  Sources are sorted
 // by patch, so each thread keeps the source interval [ea, eb) of the next near patch
@@ -219,26 +248,25 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
         T v[REC * PER];
 #pragma unroll
         for (int w = 0; w < REC; w++) Word16<T>::unpack(tile[k * REC + w], v + w * PER);
-        T er, ei, tdr, tdi, r2;
-        pair_kernel<T, GEN>(x - v[0], y - v[1], z - v[2], v[3], v[4], v[5], kr, krev, ki, er, ei, tdr, tdi, r2);
+        const T dx = x - v[0], dy = y - v[1], dz = z - v[2];
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        T rinv;
+        if constexpr (sizeof(T) == 4) rinv = __frsqrt_rn(r2);
+        else rinv = rsqrt_nr(r2);
         // with the mask, sources within thresh of the target belong to its near patches
-        const bool keep = FULL || (MASK ? jk < ea : r2 > thresh2);
-        constexpr bool SEL_FACTORS = NRHS > 2;  // zero the pair's factors once, not each sum
-        if (SEL_FACTORS) {
-          er = keep ? er : (T)0;
-          ei = keep ? ei : (T)0;
-          tdr = keep ? tdr : (T)0;
-          tdi = keep ? tdi : (T)0;
-        }
+        if (!FULL) rinv = (MASK ? jk < ea : r2 > thresh2) ? rinv : (T)0;
+        const T d = dx * v[3] + dy * v[4] + dz * v[5];
+        T c, sn, tdr, tdi;
+        pair_factors<T, GEN>(r2, rinv, d, kr, krev, ki, c, sn, tdr, tdi);
 #pragma unroll
         for (int r = 0; r < NRHS; r++) {
           const V2<T> q = {v[6 + 4 * r], v[7 + 4 * r]}, m = {v[8 + 4 * r], v[9 + 4 * r]};
-          // inner = q + m*(tdr + i tdi)
+          // inner = q + m*(tdr + i tdi); contribution = inner * (c + i sn) / r
           const T ir = q.x + m.x * tdr - m.y * tdi;
           const T ii = q.y + m.x * tdi + m.y * tdr;
-          const T cr = ir * er - ii * ei, ci = ir * ei + ii * er;
-          accr[r] += SEL_FACTORS || keep ? cr : (T)0;
-          acci[r] += SEL_FACTORS || keep ? ci : (T)0;
+          const T zr = ir * c - ii * sn, zi = ir * sn + ii * c;
+          accr[r] += zr * rinv;
+          acci[r] += zi * rinv;
         }
       }
     };
