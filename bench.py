@@ -235,11 +235,16 @@ def main():
         npts_, nptso, _, _ = plan.info()
         tm = np.zeros(4)
         s = rhs.clone()
+        s32 = s.to(torch.complex64)
         plan.apply(s, 64, 0)
-        plan.apply(s, 64, 0, times=tm)
-        t64 = tm.copy()
-        plan.apply(s.to(torch.complex64), 32, 1, times=tm)
-        t32 = tm.copy()
+        plan.apply(s32, 32, 1)
+        t64 = np.zeros(4)
+        t32 = np.zeros(4)
+        for _ in range(3):  # per-phase HIP-event times, mean of 3 warm applies each
+            plan.apply(s, 64, 0, times=tm)
+            t64 += tm / 3
+            plan.apply(s32, 32, 1, times=tm)
+            t32 += tm / 3
         # warm
         layer.gmres(plan, rhs, zid, wts, numit=200, eps_gmres=1e-6)
         torch.cuda.synchronize()
