@@ -119,54 +119,93 @@ __global__ void k_classify(int64_t ntarg, int64_t ndtarg, const double* __restri
                            int2* __restrict__ adap_list, int* __restrict__ adap_n,
                            int2* __restrict__ self_list, int* __restrict__ self_n,
                            int* __restrict__ pair_class) {
+  // Pass 1 classifies each pair of the target's row (0 self, 1 adaptive, 2 fixed rule)
+  // into pair_class and counts them; the lanes' counts are scanned within the wavefront,
+  // so that one atomic per wavefront and list reserves the wavefront's slots (the list
+  // counters are single addresses that every wavefront updates); pass 2 writes the
+  // entries.  blockDim must be a multiple of 64.
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= ntarg) return;
-  const double tx = targs[ndtarg * t], ty = targs[ndtarg * t + 1], tz = targs[ndtarg * t + 2];
-  const int64_t pid = ipatch_id ? ipatch_id[t] : -1;
-  for (int64_t j = row_ptr[t] - 1; j < row_ptr[t + 1] - 1; j++) {
+  const int lane = threadIdx.x & 63;
+  int64_t j0 = 0, j1 = 0, pid = -1;
+  double tx = 0.0, ty = 0.0, tz = 0.0;
+  if (t < ntarg) {
+    tx = targs[ndtarg * t];
+    ty = targs[ndtarg * t + 1];
+    tz = targs[ndtarg * t + 2];
+    pid = ipatch_id ? ipatch_id[t] : -1;
+    j0 = row_ptr[t] - 1;
+    j1 = row_ptr[t + 1] - 1;
+  }
+  int ns = 0, na = 0;
+  for (int64_t j = j0; j < j1; j++) {
     const int64_t ip = col_ind[j] - 1;
     int cls;
     if (pid == ip + 1) {
       cls = 0;
-      const int k = atomicAdd(self_n, 1);
-      self_list[k] = make_int2((int)j, (int)t);
+      ns++;
     } else {
       const double dx = tx - cms[3 * ip], dy = ty - cms[3 * ip + 1], dz = tz - cms[3 * ip + 2];
       const double rr = rfac0 * rads[ip];
       if (dx * dx + dy * dy + dz * dz <= rr * rr) {
         cls = 1;
-        const int k = atomicAdd(adap_n, 1);
-        adap_list[k] = make_int2((int)j, (int)t);
+        na++;
       } else {
         cls = 2;
         atomicAdd(fixed_cnt + ip, 1);
       }
     }
-    if (pair_class) pair_class[j] = cls;
+    pair_class[j] = cls;
+  }
+  int ps = ns, pa = na;  // inclusive scans over the lanes
+  for (int d = 1; d < 64; d <<= 1) {
+    const int us = __shfl_up(ps, d, 64), ua = __shfl_up(pa, d, 64);
+    if (lane >= d) {
+      ps += us;
+      pa += ua;
+    }
+  }
+  int bs = 0, ba = 0;
+  if (lane == 63) {
+    if (ps > 0) bs = atomicAdd(self_n, ps);
+    if (pa > 0) ba = atomicAdd(adap_n, pa);
+  }
+  int ks = __shfl(bs, 63, 64) + ps - ns, ka = __shfl(ba, 63, 64) + pa - na;
+  for (int64_t j = j0; j < j1; j++) {
+    const int cls = pair_class[j];
+    if (cls == 0) self_list[ks++] = make_int2((int)j, (int)t);
+    else if (cls == 1) adap_list[ka++] = make_int2((int)j, (int)t);
   }
 }
 
-__global__ void k_scan(int64_t n, const int* __restrict__ cnt, int* __restrict__ off) {
-  // single workgroup exclusive scan (n = number of patches)
-  __shared__ int part[1024];
-  const int tid = threadIdx.x;
+__global__ void __launch_bounds__(1024) k_scan(int64_t n, const int* __restrict__ cnt, int* __restrict__ off) {
+  // Single-workgroup exclusive scan, off[n] = total: each thread sums a contiguous chunk,
+  // the chunk sums are scanned within each wavefront (shuffles) and across the
+  // wavefronts (LDS), then each thread writes its chunk's offsets.
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int nw = (int)(blockDim.x >> 6);
   const int64_t chunk = (n + blockDim.x - 1) / blockDim.x;
   const int64_t a = tid * chunk, b = a + chunk < n ? a + chunk : n;
   int s = 0;
   for (int64_t i = a; i < b; i++) s += cnt[i];
-  part[tid] = s;
+  int v = s;  // inclusive scan over the lanes
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  if (lane == 63) wsum[w] = v;
   __syncthreads();
-  if (tid == 0) {
-    int acc = 0;
-    for (int i = 0; i < (int)blockDim.x; i++) {
-      const int v = part[i];
-      part[i] = acc;
-      acc += v;
+  if (w == 0) {
+    int x = lane < nw ? wsum[lane] : 0;
+    for (int d = 1; d < 16; d <<= 1) {
+      const int u = __shfl_up(x, d, 64);
+      if (lane >= d) x += u;
     }
-    off[n] = acc;
+    if (lane < nw) wsum[lane] = x;
   }
   __syncthreads();
-  int acc = part[tid];
+  int acc = v - s + (w > 0 ? wsum[w - 1] : 0);
+  if (tid == (int)blockDim.x - 1) off[n] = acc + s;
   for (int64_t i = a; i < b; i++) {
     off[i] = acc;
     acc += cnt[i];
@@ -1218,7 +1257,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
 
   // timing events, destroyed on every return path
   struct Events {
-    hipEvent_t e[14] = {};
+    hipEvent_t e[15] = {};
     ~Events() {
       for (auto& x : e)
         if (x) (void)hipEventDestroy(x);
@@ -1228,7 +1267,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   for (auto& e : evs.e) HCHK(hipEventCreate(&e));
   HCHK(hipEventRecord(ev[0], s));
   // side stream: the tree cache (on-surface targets, where adaptive pairs are all but
-  // certain: it then runs beside the classification and its host read-back), then the
+  // certain: it then runs beside the list building and its host read-back), then the
   // fixed-rule and self kernels
   hipStream_t s2 = s;
   if (g_use_side && npat <= SIDE_MAX_PATCHES) {
@@ -1246,21 +1285,24 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   double *d_tuv = TB.tuv, *d_twq = TB.twq, *d_ptab = TB.ptab;
   const bool early_tree = s2 != s && A.ipatch_id != nullptr && npat > 0;
   double* d_cache = nullptr;
-  if (early_tree) {
-    d_cache = wsalloc<double>(W_CACHE, (size_t)npat * CACHE_STRIDE * ncn);
-    if (!d_cache) return -2;
-    HCHK(hipStreamWaitEvent(s2, ev[0], 0));
-    HCHK(hipEventRecord(ev[9], s2));
-    k_tree_cache<N><<<npat, 256, 0, s2>>>(npat, kq, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs, d_cache);
-    HCHK(hipGetLastError());
-    HCHK(hipEventRecord(ev[8], s2));
-  }
   const int tb = 256;
   k_patch_prep<N><<<(npat + tb - 1) / tb, tb, 0, s>>>(npat, A.ixyzs, A.srccoefs, A.srcvals, d_rvw, eps_adap,
                                                       d_cms, d_rads, d_epsp, d_area);
   k_classify<<<(A.ntarg + tb - 1) / tb, tb, 0, s>>>(A.ntarg, A.ndtarg, A.targs, A.ipatch_id, A.row_ptr,
                                                     A.col_ind, d_cms, d_rads, A.rfac0, d_cnt, d_adap,
                                                     d_counts + 0, d_self, d_counts + 1, d_cls);
+  // The tree cache starts after the two small kernels above (it fills every CU and would
+  // hold them back) and runs beside the list building and its host read-back.
+  if (early_tree) {
+    d_cache = wsalloc<double>(W_CACHE, (size_t)npat * CACHE_STRIDE * ncn);
+    if (!d_cache) return -2;
+    HCHK(hipEventRecord(ev[14], s));
+    HCHK(hipStreamWaitEvent(s2, ev[14], 0));
+    HCHK(hipEventRecord(ev[9], s2));
+    k_tree_cache<N><<<npat, 256, 0, s2>>>(npat, kq, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs, d_cache);
+    HCHK(hipGetLastError());
+    HCHK(hipEventRecord(ev[8], s2));
+  }
   k_scan<<<1, 1024, 0, s>>>(npat, d_cnt, d_off);
   k_scatter_fixed<<<(A.ntarg + tb - 1) / tb, tb, 0, s>>>(A.ntarg, A.row_ptr, A.col_ind, d_cls, d_off, d_fill,
                                                          d_fixed);
