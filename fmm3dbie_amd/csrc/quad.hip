@@ -21,6 +21,7 @@
 // column c, then weights = umat^T * moments.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <map>
@@ -1122,22 +1123,54 @@ std::map<std::tuple<int, int, int, int, int>, OrderTables> g_tables;
     }                                                                             \
   } while (0)
 
-bool g_const_ready = false;
+// Constant-memory tables are per device: uploaded once per device (Koornwinder
+// constants) and when the self rule size changes (its Gauss-Legendre rule).
+std::map<int, int> g_const_dev;  // device -> self rule size last uploaded (0: none)
 
 int ensure_constants() {
-  if (g_const_ready) return 0;
+  int dev = 0;
+  HCHK(hipGetDevice(&dev));
+  if (g_const_dev.count(dev)) return 0;
   KoornConst kc;
   fmmhost::fill_koorn_const(&kc);
   HCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_koorn), &kc, sizeof(kc)));
-  g_const_ready = true;
+  g_const_dev[dev] = 0;
   return 0;
 }
 
 int set_self_gl(int n) {
+  int dev = 0;
+  HCHK(hipGetDevice(&dev));
+  auto it = g_const_dev.find(dev);
+  if (it != g_const_dev.end() && it->second == n) return 0;
   GLRule gl = {};
   fmmhost::gauss_legendre01(n, gl.x, gl.w);
   HCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_self_gl), &gl, sizeof(gl)));
+  g_const_dev[dev] = n;
   return 0;
+}
+
+// Timing events of the generation, created once per device and reused: every call
+// records each of them before reading any, and synchronises before it returns.
+constexpr int NQ_EVENTS = 15;
+std::map<int, std::array<hipEvent_t, NQ_EVENTS>> g_events;
+
+hipEvent_t* quad_events() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  auto it = g_events.find(dev);
+  if (it == g_events.end()) {
+    std::array<hipEvent_t, NQ_EVENTS> e{};
+    for (auto& x : e) {
+      if (hipEventCreate(&x) != hipSuccess) {
+        for (auto& y : e)
+          if (y) (void)hipEventDestroy(y);
+        return nullptr;
+      }
+    }
+    it = g_events.emplace(dev, e).first;
+  }
+  return it->second.data();
 }
 
 template <typename T>
@@ -1255,16 +1288,11 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   kp.ar = A.alpha[0]; kp.ai = A.alpha[1];
   kp.br = A.beta[0]; kp.bi = A.beta[1];
 
-  // timing events, destroyed on every return path
-  struct Events {
-    hipEvent_t e[15] = {};
-    ~Events() {
-      for (auto& x : e)
-        if (x) (void)hipEventDestroy(x);
-    }
-  } evs;
-  hipEvent_t* ev = evs.e;
-  for (auto& e : evs.e) HCHK(hipEventCreate(&e));
+  hipEvent_t* ev = quad_events();
+  if (!ev) {
+    fprintf(stderr, "fmm3dbie_amd: event creation failed in getnearquad\n");
+    return -1;
+  }
   HCHK(hipEventRecord(ev[0], s));
   // side stream: the tree cache (on-surface targets, where adaptive pairs are all but
   // certain: it then runs beside the list building and its host read-back), then the
@@ -1509,7 +1537,10 @@ int fmm_nearquad_device(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s)
   return A.helm ? dispatch<true>(A, st, s) : dispatch<false>(A, st, s);
 }
 
-int fmm_koorn_constants() { return ensure_constants(); }
+int fmm_koorn_constants() {
+  std::lock_guard<std::mutex> lock(g_quad_mutex);
+  return ensure_constants();
+}
 
 void fmm_nearquad_set_side_stream(int enable) {
   std::lock_guard<std::mutex> lock(g_quad_mutex);
@@ -1527,6 +1558,14 @@ void fmm_nearquad_release() {
     (void)hipSetDevice(cur);
   }
   g_side.clear();
+  for (auto& kv : g_events) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(kv.first);
+    for (auto& e : kv.second) (void)hipEventDestroy(e);
+    (void)hipSetDevice(cur);
+  }
+  g_events.clear();
   for (auto& kv : g_ws)
     for (auto& b : kv.second)
       if (b.p) (void)hipFree(b.p);
