@@ -1246,6 +1246,28 @@ int build_tables(int nqorder, int nlev, int nqorder_f, OrderTables& T, hipStream
   return 0;
 }
 
+// Persistent wavefronts of the adaptive kernel: one per resident slot on this device
+// (CUs x workgroups per CU at the kernel's register and LDS use); more would only wait
+// for a slot and find the queue drained, fewer leave slots to the side-stream kernels
+// but slow the kernel alone.
+template <int N, bool HELM>
+int adaptive_slots() {
+  static std::map<int, int> slots;  // per device
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  auto it = slots.find(dev);
+  if (it != slots.end()) return it->second;
+  int ncu = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adaptive<N, HELM>, 64, 0) != hipSuccess)
+    return 0;
+  int n = ncu * per_cu;
+  if (n < 64) n = 64;
+  if (n > 8192) n = 8192;
+  slots[dev] = n;
+  return n;
+}
+
 template <int N, bool HELM>
 int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
@@ -1370,7 +1392,11 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     if (!d_cache) return -2;
   }
   // adaptive
-  const int nwaves = A.adaptive_waves > 0 ? A.adaptive_waves : 4096;
+  const int nwaves = A.adaptive_waves > 0 ? A.adaptive_waves : adaptive_slots<N, HELM>();
+  if (nwaves <= 0) {
+    fprintf(stderr, "fmm3dbie_amd: occupancy query for the adaptive kernel failed\n");
+    return -1;
+  }
   constexpr int ES = 6 + 2 * NP;
   double* d_stack = wsalloc<double>(W_STACK, (size_t)nwaves * STACK_CAP * ES);
   if (!d_stack) return -2;
