@@ -850,6 +850,10 @@ static int far_and_csr(ApplyPlan* P, int prec, bool mask, const V2<T>* q, const 
 int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* pot, int nrhs, hipStream_t s,
                    ApplyTimes* tm) {
   if (nrhs < 1 || nrhs > 64) return -6;
+  if (prec != 32 && prec != 64) {
+    fprintf(stderr, "fmm3dbie_amd: apply precision %d (must be 32 or 64)\n", prec);
+    return -6;
+  }
   if (ensure_ws(P, nrhs) != 0) return -2;
   const int64_t N = P->npts, NT = P->ntarg, NO = P->nptso;
   const double ar = P->alpha[0], ai = P->helm ? P->alpha[1] : 0.0;

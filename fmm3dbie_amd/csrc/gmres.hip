@@ -190,6 +190,25 @@ __global__ void k_gather(int64_t n, const C2<T>* __restrict__ V, int64_t ldvr, C
 
 inline dim3 g2(int64_t n, int r) { return dim3((unsigned)((n + 255) / 256), (unsigned)r); }
 
+// Device buffers of one solve, freed on every return path (early error returns included).
+struct DevBufs {
+  std::vector<void*> p;
+  template <typename T>
+  hipError_t alloc(T** out, size_t bytes) {
+    *out = nullptr;
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, bytes ? bytes : 1);
+    if (e == hipSuccess) {
+      p.push_back(q);
+      *out = (T*)q;
+    }
+    return e;
+  }
+  ~DevBufs() {
+    for (void* q : p) (void)hipFree(q);
+  }
+};
+
 #define GCHK(x)                                                                    \
   do {                                                                             \
     hipError_t e_ = (x);                                                           \
@@ -211,13 +230,14 @@ int gmres_impl(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, voi
   const int64_t ldv = N, ldvr = N * (int64_t)(numit + 1);
   C2<T>*V = nullptr, *W = nullptr, *KX = nullptr;
   double* dh = nullptr;
-  GCHK(hipMalloc((void**)&V, sizeof(C2<T>) * ldvr * nrhs));
-  GCHK(hipMalloc((void**)&W, sizeof(C2<T>) * N * nrhs));
-  GCHK(hipMalloc((void**)&KX, sizeof(C2<T>) * N * nrhs));
+  DevBufs bufs;
+  GCHK(bufs.alloc(&V, sizeof(C2<T>) * ldvr * nrhs));
+  GCHK(bufs.alloc(&W, sizeof(C2<T>) * N * nrhs));
+  GCHK(bufs.alloc(&KX, sizeof(C2<T>) * N * nrhs));
   // dot products of the two Gram-Schmidt passes and the norm, each 2 (numit + 2) nrhs
   // doubles, read back together once per iteration; then the scale factors
   const int64_t LH = 2 * (int64_t)(numit + 2) * nrhs;
-  GCHK(hipMalloc((void**)&dh, sizeof(double) * (3 * LH + 2 * nrhs + LH * DOT_SMAX)));
+  GCHK(bufs.alloc(&dh, sizeof(double) * (3 * LH + 2 * nrhs + LH * DOT_SMAX)));
   double* dsc = dh + 3 * LH;
   double* dhp = dsc + 2 * nrhs;  // partial dot products of the row slices
   std::vector<double> hh(LH), hh2(LH), hh3(3 * LH);
@@ -342,10 +362,6 @@ int gmres_impl(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, voi
   }
   GCHK(hipStreamSynchronize(s));
   if (gs) gs->napply += napply;
-  (void)hipFree(V);
-  (void)hipFree(W);
-  (void)hipFree(KX);
-  (void)hipFree(dh);
   return 0;
 }
 
@@ -358,6 +374,11 @@ int gmres_device(ApplyPlan* P, int prec, int mode, const double* zid, const void
     fprintf(stderr, "fmm3dbie_amd: GMRES needs the on-surface operator (plan targets = discretization points)\n");
     return -6;
   }
+  if (prec != 32 && prec != 64) {
+    fprintf(stderr, "fmm3dbie_amd: GMRES precision %d (must be 32 or 64)\n", prec);
+    return -6;
+  }
+  if (nrhs < 1 || nrhs > 64 || numit < 1) return -6;
   if (prec == 64)
     return gmres_impl<double>(P, mode, zid, rhs, soln, wts, nrhs, numit, eps_gmres, niter, errs, rres, s, gs);
   return gmres_impl<float>(P, mode, zid, rhs, soln, wts, nrhs, numit, eps_gmres, niter, errs, rres, s, gs);
@@ -372,17 +393,19 @@ int gmres_ir_device(ApplyPlan* P, int mode, const double* zid, const void* rhs_v
     fprintf(stderr, "fmm3dbie_amd: GMRES needs the on-surface operator (plan targets = discretization points)\n");
     return -6;
   }
+  if (nrhs < 1 || nrhs > 64 || numit < 1 || max_outer < 1) return -6;
   const int64_t N = P->npts;
   const C2<double>* rhs = (const C2<double>*)rhs_v;
   C2<double>* x = (C2<double>*)soln_v;
   C2<double>*r64 = nullptr, *kx = nullptr;
   C2<float>*r32 = nullptr, *c32 = nullptr;
   double* dh = nullptr;
-  GCHK(hipMalloc((void**)&r64, sizeof(C2<double>) * N * nrhs));
-  GCHK(hipMalloc((void**)&kx, sizeof(C2<double>) * N * nrhs));
-  GCHK(hipMalloc((void**)&r32, sizeof(C2<float>) * N * nrhs));
-  GCHK(hipMalloc((void**)&c32, sizeof(C2<float>) * N * nrhs));
-  GCHK(hipMalloc((void**)&dh, sizeof(double) * 4 * nrhs * (1 + DOT_SMAX)));
+  DevBufs bufs;
+  GCHK(bufs.alloc(&r64, sizeof(C2<double>) * N * nrhs));
+  GCHK(bufs.alloc(&kx, sizeof(C2<double>) * N * nrhs));
+  GCHK(bufs.alloc(&r32, sizeof(C2<float>) * N * nrhs));
+  GCHK(bufs.alloc(&c32, sizeof(C2<float>) * N * nrhs));
+  GCHK(bufs.alloc(&dh, sizeof(double) * 4 * nrhs * (1 + DOT_SMAX)));
   double* dhp = dh + 4 * nrhs;  // partial dot products of the row slices
   GCHK(hipMemsetAsync(x, 0, sizeof(C2<double>) * N * nrhs, s));
   GCHK(hipMemcpyAsync(r64, rhs, sizeof(C2<double>) * N * nrhs, hipMemcpyDeviceToDevice, s));
@@ -393,7 +416,7 @@ int gmres_ir_device(ApplyPlan* P, int mode, const double* zid, const void* rhs_v
   std::vector<int> nit(nrhs), upd(nrhs, 1);
   std::vector<double> errs((size_t)numit * nrhs), eps_r(nrhs, eps_inner), cur(nrhs, 1.0);
   int* dupd = nullptr;
-  GCHK(hipMalloc((void**)&dupd, sizeof(int) * nrhs));
+  GCHK(bufs.alloc(&dupd, sizeof(int) * nrhs));
   int outer = 0, inner = 0;
   double worst = 1.0;
   for (outer = 1; outer <= max_outer; outer++) {
@@ -430,14 +453,10 @@ int gmres_ir_device(ApplyPlan* P, int mode, const double* zid, const void* rhs_v
     }
     if (worst <= tol) break;
   }
-  (void)hipFree(dupd);
   if (outer > max_outer) outer = max_outer;
   if (outer_out) *outer_out = outer;
   if (inner_total) *inner_total = inner;
-  (void)hipFree(r64);
-  (void)hipFree(kx);
-  (void)hipFree(r32);
-  (void)hipFree(c32);
-  (void)hipFree(dh);
+  // the buffers are freed on return; the stream's work on them is complete (the last
+  // read-back above synchronised it)
   return 0;
 }

@@ -363,6 +363,35 @@ class NearQuadDevice:
         return self.wnear
 
 
+def _dens(t, npts, prec, what):
+    """Checks a density / right-hand-side tensor handed to the kernels: on the GPU,
+    contiguous, complex128 (prec 64) or complex64 (prec 32), shape (npts,) or
+    (nrhs, npts) with 1 <= nrhs <= 64.  Returns nrhs."""
+    torch = _torch()
+    if prec not in (32, 64):
+        raise ValueError("prec must be 32 or 64, got %r" % (prec,))
+    want = torch.complex128 if prec == 64 else torch.complex64
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError("%s must be a GPU tensor" % what)
+    if t.dtype != want:
+        raise ValueError("%s must be %s for prec %d, got %s" % (what, want, prec, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % what)
+    if t.dim() == 1 and t.shape[0] == npts:
+        return 1
+    if t.dim() == 2 and t.shape[1] == npts and 1 <= t.shape[0] <= 64:
+        return int(t.shape[0])
+    raise ValueError("%s must have shape (%d,) or (nrhs, %d) with nrhs <= 64, got %s"
+                     % (what, npts, npts, tuple(t.shape)))
+
+
+def _weights(w, npts):
+    torch = _torch()
+    if (not isinstance(w, torch.Tensor) or not w.is_cuda or w.dtype != torch.float64 or not w.is_contiguous()
+            or tuple(w.shape) != (npts,)):
+        raise ValueError("wts must be a contiguous GPU float64 tensor of shape (%d,)" % npts)
+
+
 class ApplyPlan:
     """Device-resident apply (lpcomp_*_addsub) reused across matvecs and RHS."""
 
@@ -390,10 +419,13 @@ class ApplyPlan:
             raise RuntimeError("apply plan creation failed (status %d)" % lib().fmm3dbie_last_error())
 
     def apply(self, sigma, prec=64, mode=0, stream=None, times=None):
-        """sigma: complex tensor (npts,) or (nrhs, npts); returns pot of the same shape."""
+        """sigma: complex tensor (npts,) or (nrhs, npts) on the GPU, complex128 for
+        prec 64 and complex64 for prec 32; returns pot of the same shape."""
         torch = _torch()
-        sig = sigma.contiguous()
-        nrhs = 1 if sig.dim() == 1 else sig.shape[0]
+        if mode not in (0, 1):
+            raise ValueError("mode must be 0 (add/subtract) or 1 (masked)")
+        nrhs = _dens(sigma, self.npts, prec, "sigma")
+        sig = sigma
         pot = torch.empty_like(sig)
         tm = np.zeros(4)
         s = stream if stream is not None else torch.cuda.current_stream()
@@ -439,8 +471,11 @@ def gmres(plan, rhs, zid, wts, numit=200, eps_gmres=1e-7, prec=64, mode=0, strea
     """GMRES for (zid I + K) x = rhs on the device.  rhs (npts,) or (nrhs, npts) complex.
     Returns (x, niter[nrhs], errs[nrhs, numit], rres[nrhs], applies)."""
     torch = _torch()
-    r = rhs.contiguous()
-    nrhs = 1 if r.dim() == 1 else r.shape[0]
+    nrhs = _dens(rhs, plan.npts, prec, "rhs")
+    _weights(wts, plan.npts)
+    if mode not in (0, 1) or int(numit) < 1:
+        raise ValueError("mode must be 0 or 1 and numit >= 1")
+    r = rhs
     x = torch.empty_like(r)
     niter = np.zeros(nrhs, np.int32)
     errs = np.zeros((nrhs, numit))
@@ -459,8 +494,11 @@ def gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=1e-4, tol=1e-6, max_outer
     """Mixed precision: FP32 inner GMRES + FP64 residual refinement.
     Returns (x, outer, inner_total, rres[nrhs], fp32 applies, fp64 applies)."""
     torch = _torch()
-    r = rhs.contiguous()
-    nrhs = 1 if r.dim() == 1 else r.shape[0]
+    nrhs = _dens(rhs, plan.npts, 64, "rhs")
+    _weights(wts, plan.npts)
+    if mode not in (0, 1) or int(numit) < 1 or int(max_outer) < 1:
+        raise ValueError("mode must be 0 or 1, numit >= 1 and max_outer >= 1")
+    r = rhs
     x = torch.empty_like(r)
     outer = C.c_int(0)
     inner = C.c_int(0)

@@ -20,6 +20,8 @@ def setup(geom, eps, helm, pars, far_order=True, nover_fixed=None, gpu_near=Fals
     row_ptr, col_ind = finder(cms, rads * rfac, targs)
     iquad = near.get_iquad_rsc(geom, row_ptr, col_ind)
     if helm:
+        # the complex wavenumber zpars(1), as the reference passes it to get_far_order
+        # (helm_comb_dir.f:596, 2010)
         zk = complex(np.asarray(pars, np.complex128)[0])
         beta = complex(np.asarray(pars, np.complex128)[2])
     else:
@@ -30,7 +32,7 @@ def setup(geom, eps, helm, pars, far_order=True, nover_fixed=None, gpu_near=Fals
         novers = np.full(len(geom["norders"]), int(nover_fixed), np.int64)
     elif far_order:
         far = near.get_far_order_gpu if gpu_near else near.get_far_order
-        novers, _ = far(eps, geom, cms, rads, targs, ikerorder, zk.real if helm else 0.0, row_ptr, col_ind, rfac)
+        novers, _ = far(eps, geom, cms, rads, targs, ikerorder, zk if helm else 0.0, row_ptr, col_ind, rfac)
     else:
         novers = np.asarray(geom["norders"], np.int64).copy()
     srcover, whtsover, ixyzso = near.oversample_geom(geom, novers)

@@ -282,3 +282,135 @@ def test_oracle_gmres_small_dense():
     xd = np.linalg.solve(A, rhs)
     assert np.abs(x - xd).max() < 1e-9 * np.abs(xd).max()
     assert rres < 1e-10
+
+
+# ---------------------------------------------------------------- far-field order pins
+# get_far_order_guru (src/quadratures/far_field_routs.f90:300-532): the moments of the
+# 25 probes are compared between successive RV orders of {1,2,3,4,6,8,10,12,14,16}; at
+# the first agreement (l2 over the moments, max over the probes, below
+# rsc^(1/4) eps) the lower order of the pair is kept (:520-527); without agreement the
+# loop ends with nnn = 11 and the order is 16.
+
+_NFARS = [1, 2, 3, 4, 6, 8, 10, 12, 14, 16]
+
+
+def _flat_patch(norder, A, B, C):
+    """One planar triangle x(u,v) = A + u (B-A) + v (C-A) sampled at the RV nodes."""
+    from fmm3dbie_amd import geometry as G
+    uv, _ = O.rv_nodes(norder)
+    A, B, C = (np.asarray(x, float) for x in (A, B, C))
+    du, dv = B - A, C - A
+    n = np.cross(du, dv)
+    n /= np.linalg.norm(n)
+    sv = np.zeros((len(uv), 12))
+    sv[:, :3] = A[None, :] + uv[:, :1] * du[None, :] + uv[:, 1:] * dv[None, :]
+    sv[:, 3:6] = du
+    sv[:, 6:9] = dv
+    sv[:, 9:12] = n
+    return G.geometry_from_srcvals(norder, 1, sv), np.linalg.norm(np.cross(du, dv))
+
+
+def _duffy_moments(probes, A, B, C, norder, kernel, zk, nq=48):
+    """Moments of every probe against the Koornwinder basis on the flat triangle, by a
+    tensor Gauss-Legendre rule on the square mapped to the simplex (u = s, v = s t, with
+    Jacobian s): an integrator independent of the RV and XG tables.  The probes are
+    well separated from the triangle, so the integrand is smooth and nq = 48 gives the
+    integrals to rounding."""
+    x, w = np.polynomial.legendre.leggauss(nq)
+    x = 0.5 * (x + 1)
+    w = 0.5 * w
+    s, t = np.meshgrid(x, x, indexing="ij")
+    ws = (w[:, None] * w[None, :]) * s
+    u, v = s * (1 - t), s * t          # (u, v) covers the simplex u, v >= 0, u + v <= 1
+    u, v, ws = u.ravel(), v.ravel(), ws.ravel()
+    A, B, C = (np.asarray(q, float) for q in (A, B, C))
+    du, dv = B - A, C - A
+    J = np.linalg.norm(np.cross(du, dv))
+    nrm = np.cross(du, dv) / J
+    y = A[None, :] + u[:, None] * du[None, :] + v[:, None] * dv[None, :]
+    Pm = np.stack([O.koorn_pols(a, b, norder) for a, b in zip(u, v)])  # (q, p)
+    d = probes[:, None, :] - y[None, :, :]
+    r = np.linalg.norm(d, axis=-1)
+    e = np.exp(1j * zk * r) / (4 * np.pi)
+    if kernel == "slp":
+        val = e / r
+    else:
+        val = (d @ nrm) * (1 - 1j * zk * r) * e / r ** 3
+    return (val * (ws * J)[None, :]) @ Pm  # (probe, p)
+
+
+def _far_probes(cm, rad, near_targets):
+    """The 25 probes of get_far_order_guru (far_field_routs.f90:338-357): the
+    min(10, n/2) near-list targets farthest from the centroid, then points on the sphere
+    of radius rad about the centroid at angles from hkrand(1001 + i), hkrand(3042 + i)."""
+    d = ((near_targets - cm[None, :]) ** 2).sum(1)
+    order = np.argsort(d, kind="stable")
+    i0 = min(10, len(near_targets) // 2)
+    pr = [near_targets[order[len(order) - 1 - i]] for i in range(i0)]
+    for i in range(i0, 25):
+        phi = O.hkrand(1001 + i + 1) * 2 * np.pi
+        th = O.hkrand(3042 + i + 1) * np.pi
+        pr.append(cm + rad * np.array([np.sin(th) * np.cos(phi), np.sin(th) * np.sin(phi), np.cos(th)]))
+    return np.array(pr)
+
+
+def _one_patch_far_order(g, eps, kernel, zk, targs, rfac=2.0):
+    cms, rads = O.centroid_rads(g["norders"], g["ixyzs"], g["srccoefs"])
+    nt = len(targs)
+    row_ptr = np.arange(nt + 1, dtype=np.int64) + 1
+    col_ind = np.ones(nt, np.int64)
+    nv, ix = O.far_order(eps, g, cms, rads, targs, -1 if kernel == "slp" else 0, zk, row_ptr, col_ind, rfac)
+    return int(nv[0]), cms[0], rads[0]
+
+
+@pytest.mark.parametrize("kernel,zk", [("slp", 0.0), ("dlp", 2.0)])
+def test_far_order_limits(kernel, zk):
+    """eps = 1e30: orders 1 and 2 already agree, and the lower one (1) is kept; eps =
+    1e-30: no pair of orders agrees and the order is 16."""
+    A, B, C = [0.0, 0.0, 0.0], [0.3, 0.05, 0.0], [0.1, 0.25, 0.02]
+    g, _ = _flat_patch(4, A, B, C)
+    targs = np.array([[0.2, 0.1, 0.3], [0.6, 0.4, -0.1], [-0.2, 0.3, 0.2], [0.1, -0.3, 0.1]])
+    assert _one_patch_far_order(g, 1e30, kernel, zk, targs)[0] == 1
+    assert _one_patch_far_order(g, 1e-30, kernel, zk, targs)[0] == 16
+
+
+@pytest.mark.parametrize("kernel,zk,norder", [("slp", 0.0, 4), ("dlp", 1.5, 3), ("slp", 4.0, 6)])
+def test_far_order_against_independent_integrals(kernel, zk, norder):
+    """The order chosen is the lowest RV order whose probe moments are within the
+    tolerance of the exact integrals (computed by an independent product rule), when
+    the tolerance is set midway (geometrically) between two successive orders' errors."""
+    A, B, C = [0.0, 0.0, 0.0], [0.3, 0.05, 0.0], [0.1, 0.25, 0.02]
+    g, J = _flat_patch(norder, A, B, C)
+    targs = np.array([[0.2, 0.1, 0.3], [0.6, 0.4, -0.1], [-0.2, 0.3, 0.2], [0.1, -0.3, 0.1],
+                      [0.5, 0.5, 0.2], [-0.3, -0.2, -0.2]])
+    cms, rads = O.centroid_rads(g["norders"], g["ixyzs"], g["srccoefs"])
+    probes = _far_probes(cms[0], 2.0 * rads[0], targs)
+    exact = _duffy_moments(probes, A, B, C, norder, kernel, zk)
+    # the RV-rule moments of each candidate order on the same flat patch
+    du, dv = np.subtract(B, A), np.subtract(C, A)
+    nrm = np.cross(du, dv) / J
+    errs = []
+    for nf in _NFARS:
+        uv, w = O.rv_nodes(nf)
+        y = np.asarray(A)[None, :] + uv[:, :1] * du[None, :] + uv[:, 1:] * dv[None, :]
+        Pm = np.stack([O.koorn_pols(a, b, norder) for a, b in uv])
+        d = probes[:, None, :] - y[None, :, :]
+        r = np.linalg.norm(d, axis=-1)
+        e = np.exp(1j * zk * r) / (4 * np.pi)
+        val = e / r if kernel == "slp" else (d @ nrm) * (1 - 1j * zk * r) * e / r ** 3
+        mom = (val * (w * J)[None, :]) @ Pm
+        errs.append(np.sqrt((np.abs(mom - exact) ** 2).sum(1)).max())
+    errs = np.array(errs)
+    # every adjacent pair (k-1, k) with a clear gap (ratio >= 50) and errors above
+    # rounding: tolerance sqrt(e_{k-1} e_k) must select order k (the first accurate one)
+    checked = 0
+    for k in range(1, 9):
+        if errs[k] < 1e-13 or errs[k - 1] / errs[k] < 50 or np.any(errs[k + 1:] > errs[k]):
+            continue
+        if np.any(errs[:k] < errs[k - 1]):
+            continue
+        epsp = np.sqrt(errs[k - 1] * errs[k])
+        eps = epsp / np.sqrt(J)  # rsc^(1/4) = sqrt(|x_u x x_v|) on a flat patch
+        assert _one_patch_far_order(g, eps, kernel, zk, targs)[0] == _NFARS[k], (k, errs)
+        checked += 1
+    assert checked >= 2, errs
