@@ -984,6 +984,44 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
   const double vx[3] = {0.0, 1.0, 0.0}, vy[3] = {0.0, 0.0, 1.0};
   Mom acc{0.0, 0.0};
   const int nn = nr * nt;
+  // Principal value over a disk about the target in the conformal frame (the reference's
+  // GGQ self rule, ggq-selfquad-routs.f90:16-214: frame from the QR factor of (x_u, x_v),
+  // disk radius rcut = half the distance to the nearest edge line).  On each ray of the
+  // product rule the radial Gauss rule gives G(0) C_GL for the 1/rho part of the
+  // integrand (C_GL = sum_i w_i / rho_i), which the disk convention integrates from
+  // rcut/R to 1: every ray adds G(0) (ln(R / rcut) - C_GL).  G(0) is the double layer's
+  // 1/r^2 coefficient, nonzero when the derivative data are not the derivatives of the
+  // xyz expansion: -beta (D.n0) / (4 pi |D|^3) P_j(uv0) |J0| (ray Jacobian), D = grad X
+  // along the ray.  grad X comes from an 11-point central difference (exact for the
+  // expansion's degree <= 10); lanes 0..19 take one stencil point each.
+  __shared__ double fd[20][3];
+  __shared__ double pj[NP];
+  const bool has_dl = kp.br != 0.0 || kp.bi != 0.0;
+  double gx[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  if (has_dl) {
+    if (lane < 20) {
+      const int dir = lane / 10, k = 1 + (lane % 10) / 2, sgn = (lane & 1) ? 1 : -1;
+      constexpr double hfd = 0.1;
+      const double cfd = k == 1 ? 5.0 / 6 : (k == 2 ? -5.0 / 21 : (k == 3 ? 5.0 / 84 : (k == 4 ? -5.0 / 504 : 1.0 / 1260)));
+      double Pf[NP], gf[12];
+      koorn_eval<N>(u0 + (dir == 0 ? sgn * k * hfd : 0.0), v0 + (dir == 1 ? sgn * k * hfd : 0.0), Pf);
+      geom_eval<NP>(sc, Pf, gf);
+      for (int m = 0; m < 3; m++) fd[lane][m] = sgn * cfd * gf[m] / hfd;
+    }
+    if (lane < NP) {
+      double Pa[NP];
+      koorn_eval<N>(u0, v0, Pa);
+#pragma unroll
+      for (int j = 0; j < NP; j++)
+        if (j == lane) pj[j] = Pa[j];
+    }
+    __syncthreads();
+    for (int q = 0; q < 20; q++)
+      for (int m = 0; m < 3; m++) gx[q / 10][m] += fd[q][m];
+  }
+  double cgl = 0.0;
+  for (int i = 0; i < nr; i++) cgl += g_self_gl.w[i] / g_self_gl.x[i];
+  double pvs = 0.0;  // this lane's rays: sum of wtau (ray Jacobian) kappa (ln(R / rcut) - C_GL)
   const float inv_nr = 1.0f / (float)nr;
   // The three triangles at the target, each split at the foot of the perpendicular: up to
   // six segments of sinh-graded panels.  Their parameters go to LDS, and the nodes of all
@@ -991,8 +1029,10 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
   // panel: a 100-node panel would leave 28 lanes of its second pass idle).
   __shared__ double segd[6][8];  // fx, fy, ex, ey, d, ta, tb, npan
   __shared__ int segc[7];        // first node of each segment; segc[nseg] = node count
+  __shared__ double sh_rcut;
   if (lane == 0) {
     int ns = 0, cnt = 0;
+    double hmin = 1e300;
     for (int e = 0; e < 3; e++) {
       const double Ax = vx[e] - u0, Ay = vy[e] - v0;
       const double Bx = vx[(e + 1) % 3] - u0, By = vy[(e + 1) % 3] - v0;
@@ -1003,6 +1043,7 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
       const double ts = -(ax * ex + ay * ey);
       const double fx = ax + ts * ex, fy = ay + ts * ey;
       const double d = sqrt(fx * fx + fy * fy);
+      hmin = d < hmin ? d : hmin;
       const double s0 = -ts, s1 = len - ts;
       double segs[3];
       int nseg = 0;
@@ -1022,9 +1063,11 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
     }
     segc[ns] = cnt;
     for (int k = ns + 1; k < 7; k++) segc[k] = cnt;
+    sh_rcut = 0.5 * hmin;
   }
   __syncthreads();
   const int total = segc[6];
+  const double rcut = sh_rcut;
   for (int base = 0; base < total; base += WAVE) {
     const int k = base + lane;
     if (k < total) {
@@ -1046,6 +1089,14 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
       const double sh = 0.5 * (et - eti), ch = 0.5 * (et + eti);
       const double wtau = g_self_gl.w[it] * (t1 - t0);
       const double px = fx + d * sh * ex, py = fy + d * sh * ey;
+      if (has_dl && ir == 0) {
+        const double dv0 = py / l22, du0 = (px - l12 * dv0) / l11;
+        const double Dx = gx[0][0] * du0 + gx[1][0] * dv0, Dy = gx[0][1] * du0 + gx[1][1] * dv0,
+                     Dz = gx[0][2] * du0 + gx[1][2] * dv0;
+        const double dn = Dx * g0[9] + Dy * g0[10] + Dz * g0[11], d2 = Dx * Dx + Dy * Dy + Dz * Dz;
+        const double kap = -dn / (4.0 * M_PI * d2 * sqrt(d2));
+        pvs += wtau * d * d * ch / detl * kap * (log(d * ch / rcut) - cgl);
+      }
       const double rho = g_self_gl.x[ir];
       const double xi0 = rho * px, xi1 = rho * py;
       const double dv = xi1 / l22, du = (xi0 - l12 * dv) / l11;
@@ -1060,6 +1111,18 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
     }
     const int nrows = total - base < WAVE ? total - base : WAVE;
     acc += column_sum<NP, HELM>(buf, nrows);
+  }
+  if (has_dl) {
+    // beta * (sum over rays) * |J0| * P_j(uv0) joins moment j (lane j)
+    const double pv = wave_sum(pvs);
+    const double jac0 = sqrt((g0[4] * g0[8] - g0[5] * g0[7]) * (g0[4] * g0[8] - g0[5] * g0[7]) +
+                             (g0[5] * g0[6] - g0[3] * g0[8]) * (g0[5] * g0[6] - g0[3] * g0[8]) +
+                             (g0[3] * g0[7] - g0[4] * g0[6]) * (g0[3] * g0[7] - g0[4] * g0[6]));
+    if (lane < NP) {
+      const double c = pv * jac0 * pj[lane];
+      acc.re += kp.br * c;
+      if (HELM) acc.im += kp.bi * c;
+    }
   }
   store_weights<NP, HELM>(buf, acc, umat, wnear, iquad[jt.x] - 1);
 }

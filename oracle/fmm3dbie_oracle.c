@@ -21,6 +21,7 @@
  *  - orc_dlaran: LAPACK's published 48-bit LCG, checked against its first outputs
  *    computed in exact integer arithmetic in the test.
  */
+#include <omp.h>
 #include "fmm3dbie_oracle.h"
 
 #include <complex.h>
@@ -101,6 +102,12 @@ void orc_koorn_pols(double u, double v, int nmax, double *pols) {
       double sc = sqrt(1.0 / (2 * k + 1) / (2 * n + 2));
       pols[iii++] = leg[k] * jac[n - k][k] * (1 / sc);
     }
+}
+
+/* orc_koorn_pols at npts points uv (2, npts) -> pols (npols, npts) */
+void orc_koorn_pols_many(int nmax, int64_t npts, const double *uv, double *pols) {
+  const int np = (nmax + 1) * (nmax + 2) / 2;
+  for (int64_t k = 0; k < npts; k++) orc_koorn_pols(uv[2 * k], uv[2 * k + 1], nmax, pols + (size_t)np * k);
 }
 
 /* get_vioreanu_nodes/wts: src/tria_routs/koornexps.f90:57-154 (table data) */
@@ -493,8 +500,57 @@ void orc_self_polar(int norder, int npols, const double *uv, const double *srcco
   double xr[64], wr[64], xt[64], wt[64];
   gauss_legendre01(nr, xr, wr);
   gauss_legendre01(nt, xt, wt);
-  double complex *fint = calloc(npols, sizeof(double complex));
+  /* principal value over a disk about the target in the conformal frame (the convention
+   * of the reference's GGQ self rule, ggq-selfquad-routs.f90:16-214: frame from the QR
+   * factor of (x_u, x_v) at the target, disk radius rcut = half the distance to the
+   * nearest edge line).  On a ray from the target to the edge point at distance R the
+   * integrand times the area Jacobian is G(rho)/rho in the ray fraction rho; the radial
+   * Gauss rule gives G(0) C_GL, C_GL = sum_i w_i / rho_i, for its 1/rho part, which the
+   * disk convention integrates from rcut/R to 1, so each ray adds G(0) (ln(R / rcut) -
+   * C_GL).  G(0) is nonzero only for the double layer when the derivative data differ
+   * from the derivatives of the xyz expansion (every interpolated geometry):
+   *   G(0) = -beta (D.n0) / (4 pi |D|^3) P_j(uv0) |J0| x (ray Jacobian),
+   * D = grad X(uv0) . Delta, Delta = the ray's uv displacement at rho = 1, n0 and |J0|
+   * the unit normal and area factor of the derivative data at the target.  grad X, the
+   * gradient of the degree-norder xyz expansion, comes from an 11-point central
+   * difference, exact for polynomials of degree <= 10. */
+  double cgl = 0;
+  for (int i = 0; i < nr; i++) cgl += wr[i] / xr[i];
+  double complex beta = 0;
+  if (kind == ORC_HELM_COMB) beta = zpars[4] + I * zpars[5];
+  else if (kind == ORC_HELM_DLP || kind == ORC_LAP_DLP) beta = 1;
+  else if (kind == ORC_LAP_COMB) beta = zpars[1];
+  double gx[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  if (beta != 0) {
+    static const double cfd[5] = {5.0 / 6, -5.0 / 21, 5.0 / 84, -5.0 / 504, 1.0 / 1260};
+    const double hfd = 0.1;
+    for (int dir = 0; dir < 2; dir++)
+      for (int k = 1; k <= 5; k++)
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+          double uu = uv[0] + (dir == 0 ? sgn * k * hfd : 0), vv = uv[1] + (dir == 1 ? sgn * k * hfd : 0);
+          orc_koorn_pols(uu, vv, norder, pols);
+          geom_at(npols, srccoefs, pols, s12);
+          for (int m = 0; m < 3; m++) gx[dir][m] += sgn * cfd[k - 1] * s12[m] / hfd;
+        }
+  }
+  double jac0 = sqrt((g0[4] * g0[8] - g0[5] * g0[7]) * (g0[4] * g0[8] - g0[5] * g0[7]) +
+                     (g0[5] * g0[6] - g0[3] * g0[8]) * (g0[5] * g0[6] - g0[3] * g0[8]) +
+                     (g0[3] * g0[7] - g0[4] * g0[6]) * (g0[3] * g0[7] - g0[4] * g0[6]));
   double verts[3][2] = {{0, 0}, {1, 0}, {0, 1}};
+  double rcut = 1e300;
+  for (int e = 0; e < 3; e++) {
+    double A[2] = {verts[e][0] - uv[0], verts[e][1] - uv[1]};
+    double B[2] = {verts[(e + 1) % 3][0] - uv[0], verts[(e + 1) % 3][1] - uv[1]};
+    double a[2] = {l11 * A[0] + l12 * A[1], l22 * A[1]};
+    double b[2] = {l11 * B[0] + l12 * B[1], l22 * B[1]};
+    double ab[2] = {b[0] - a[0], b[1] - a[1]};
+    double len = sqrt(ab[0] * ab[0] + ab[1] * ab[1]);
+    double h = fabs(a[0] * ab[1] - a[1] * ab[0]) / len;
+    if (h < rcut) rcut = h;
+  }
+  rcut *= 0.5;
+  double complex pvsum = 0; /* sum over rays of the disk correction per unit P_j(uv0) |J0| */
+  double complex *fint = calloc(npols, sizeof(double complex));
   for (int e = 0; e < 3; e++) {
     double A[2] = {verts[e][0] - uv[0], verts[e][1] - uv[1]};
     double B[2] = {verts[(e + 1) % 3][0] - uv[0], verts[(e + 1) % 3][1] - uv[1]};
@@ -523,6 +579,17 @@ void orc_self_polar(int norder, int npols, const double *uv, const double *srcco
         double sh = sinh(tau), ch = cosh(tau);
         double wtau = wt[it] * (t1 - t0);
         double px = f[0] + d * sh * ehat[0], py = f[1] + d * sh * ehat[1];
+        if (beta != 0) {
+          double dv0 = py / l22, du0 = (px - l12 * dv0) / l11;
+          double D[3], dn = 0, d2 = 0;
+          for (int m = 0; m < 3; m++) {
+            D[m] = gx[0][m] * du0 + gx[1][m] * dv0;
+            dn += D[m] * g0[9 + m];
+            d2 += D[m] * D[m];
+          }
+          double kap = -dn / (4 * M_PI * d2 * sqrt(d2));
+          pvsum += wtau * d * d * ch / detl * kap * (log(d * ch / rcut) - cgl);
+        }
         for (int ir = 0; ir < nr; ir++) {
           double rho = xr[ir];
           double xi0 = rho * px, xi1 = rho * py;
@@ -538,6 +605,10 @@ void orc_self_polar(int norder, int npols, const double *uv, const double *srcco
         }
       }
     }
+  }
+  if (beta != 0) {
+    orc_koorn_pols(uv[0], uv[1], norder, pols);
+    for (int j = 0; j < npols; j++) fint[j] += beta * pvsum * jac0 * pols[j];
   }
   umat_t_apply(npols, umat, fint, (double complex *)zquad_out);
   free(umat); free(pols); free(fint);
@@ -1119,3 +1190,8 @@ void orc_far_order(double eps, int64_t npatches, const int64_t *norders,
   for (int64_t i = 0; i < npatches; i++) ixyzso[i + 1] = ixyzso[i] + orc_npols((int)novers[i]);
   free(col_ptr); free(row_ind); free(fill);
 }
+
+/* thread count of the OpenMP loops (the CPU baseline sets it explicitly: the runtime may
+ * already have been initialised by another library in the process) */
+void orc_set_num_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_max_threads(void) { return omp_get_max_threads(); }

@@ -31,6 +31,9 @@ enum { ORC_SELF_ADAP = 0, ORC_SELF_POLAR = 1 };
 
 int orc_npols(int norder);
 void orc_koorn_pols(double u, double v, int nmax, double *pols);
+void orc_koorn_pols_many(int nmax, int64_t npts, const double *uv, double *pols);
+void orc_set_num_threads(int n);
+int orc_max_threads(void);
 int orc_rv_nodes(int norder, double *uvs, double *wts);
 int orc_rv_umat(int norder, double *umat);
 int orc_xg_rule(int deg, double *uvs, double *wts);

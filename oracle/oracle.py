@@ -41,6 +41,9 @@ def lib():
 def _sig(L):
     vp = C.c_void_p
     L.orc_koorn_pols.argtypes = [C.c_double, C.c_double, C.c_int, _d]
+    L.orc_koorn_pols_many.argtypes = [C.c_int, C.c_int64, _d, _d]
+    L.orc_set_num_threads.argtypes = [C.c_int]
+    L.orc_max_threads.restype = C.c_int
     L.orc_rv_nodes.argtypes = [C.c_int, vp, vp]
     L.orc_rv_umat.argtypes = [C.c_int, _d]
     L.orc_xg_rule.argtypes = [C.c_int, _d, _d]
@@ -95,6 +98,14 @@ def _c2d(z):
 def koorn_pols(u, v, nmax):
     out = np.zeros(npols(nmax))
     lib().orc_koorn_pols(u, v, nmax, out)
+    return out
+
+
+def koorn_pols_many(uv, nmax):
+    """(npts, npols) basis values at the rows of uv (npts, 2)."""
+    uv = _f(uv)
+    out = np.zeros((len(uv), npols(nmax)))
+    lib().orc_koorn_pols_many(nmax, len(uv), uv, out)
     return out
 
 
@@ -310,6 +321,12 @@ def far_order(eps, geom, cms, rads, targs, ikerorder, zk, row_ptr, col_ind, rfac
                         _c2d([zk]), len(col_ind), np.asarray(row_ptr, np.int64),
                         np.asarray(col_ind, np.int64), rfac, novers, ixyzso)
     return novers, ixyzso
+
+
+def set_num_threads(n):
+    """OpenMP threads of the oracle's loops (effective even after the runtime started)."""
+    lib().orc_set_num_threads(int(n))
+    return int(lib().orc_max_threads())
 
 
 def set_threads(n):
