@@ -41,6 +41,8 @@ SIGS = {
     "fmm3dbie_apply_plan_create": ([i32, i64, vp, vp, i64, vp, vp, i64, vp, vp, vp, i64, vp, i32, vp, i64, vp, vp,
                                     vp, vp], vp),
     "fmm3dbie_apply_plan_apply": ([vp, i32, i32, vp, vp, i32, vp, vp], i32),
+    "fmm3dbie_apply_plan_apply_noalloc": ([vp, i32, i32, vp, vp, i32, vp], i32),
+    "fmm3dbie_apply_scratch_bytes": ([], C.c_size_t),
     "fmm3dbie_apply_plan_info": ([vp, vp], i32),
     "fmm3dbie_apply_plan_destroy": ([vp], None),
     "fmm3dbie_apply_plan_tune": ([vp, i32, i32, vp, vp, i32], i32),

@@ -21,6 +21,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -716,7 +718,7 @@ void apply_plan_free_ws(ApplyPlan* P) {
 // buffer (far_split_cap x 64 x npts).
 template <typename T, int NRHS>
 static int launch_far(ApplyPlan* P, bool mask, int64_t wg_target, const V2<T>* q, const V2<T>* m, int64_t ldstr,
-                      int nrhs, V2<T>* part, int64_t* nsplit_out, hipStream_t s) {
+                      int nrhs, V2<T>* part, int64_t part_cap, int64_t* nsplit_out, hipStream_t s) {
   const T* const* src;
   const T* const* trg;
   if constexpr (sizeof(T) == 8) {
@@ -733,7 +735,8 @@ static int launch_far(ApplyPlan* P, bool mask, int64_t wg_target, const V2<T>* q
   const int ngrp = nrhs / NRHS;
   const int64_t ntiles = (P->nptso + TPB - 1) / TPB;
   int64_t nsplit = (wg_target + (int64_t)g * ngrp - 1) / ((int64_t)g * ngrp);
-  const int64_t cap = (int64_t)P->far_split_cap * 64 / nrhs;
+  // splits that fit the partial buffer (part_cap complex values)
+  const int64_t cap = part_cap / ((int64_t)nrhs * (P->ntarg > 0 ? P->ntarg : 1));
   if (nsplit > cap) nsplit = cap;
   if (nsplit > ntiles) nsplit = ntiles;
   if (nsplit < 1) nsplit = 1;
@@ -774,14 +777,15 @@ static int far_group(const ApplyPlan* P, int nrhs, int prec, int64_t* wg) {
 
 template <typename T>
 static int dispatch_far(ApplyPlan* P, bool mask, int grp, int64_t wg, const V2<T>* q, const V2<T>* m, int64_t ldstr,
-                        int nrhs, V2<T>* part, int64_t* nsplit, hipStream_t s) {
+                        int nrhs, V2<T>* part, int64_t part_cap, int64_t* nsplit, hipStream_t s) {
   switch (grp) {
-    case 1: return launch_far<T, 1>(P, mask, wg, q, m, ldstr, nrhs, part, nsplit, s);
-    case 2: return launch_far<T, 2>(P, mask, wg, q, m, ldstr, nrhs, part, nsplit, s);
-    case 4: return launch_far<T, 4>(P, mask, wg, q, m, ldstr, nrhs, part, nsplit, s);
-    case 8: return launch_far<T, 8>(P, mask, wg, q, m, ldstr, nrhs, part, nsplit, s);
+    case 1: return launch_far<T, 1>(P, mask, wg, q, m, ldstr, nrhs, part, part_cap, nsplit, s);
+    case 2: return launch_far<T, 2>(P, mask, wg, q, m, ldstr, nrhs, part, part_cap, nsplit, s);
+    case 4: return launch_far<T, 4>(P, mask, wg, q, m, ldstr, nrhs, part, part_cap, nsplit, s);
+    case 8: return launch_far<T, 8>(P, mask, wg, q, m, ldstr, nrhs, part, part_cap, nsplit, s);
     case 16:
-      if constexpr (sizeof(T) == 4) return launch_far<T, 16>(P, mask, wg, q, m, ldstr, nrhs, part, nsplit, s);
+      if constexpr (sizeof(T) == 4)
+        return launch_far<T, 16>(P, mask, wg, q, m, ldstr, nrhs, part, part_cap, nsplit, s);
       return -6;
     default: return -6;
   }
@@ -790,16 +794,15 @@ static int dispatch_far(ApplyPlan* P, bool mask, int grp, int64_t wg, const V2<T
 // pot = far (+ CSR) for all right-hand sides
 template <typename T>
 static int far_and_csr(ApplyPlan* P, int prec, bool mask, const V2<T>* q, const V2<T>* m, const V2<T>* sg,
-                       V2<T>* pt, int nrhs, hipStream_t s, hipEvent_t ev_far) {
+                       V2<T>* pt, int nrhs, V2<T>* part, int64_t part_cap, hipStream_t s, hipEvent_t ev_far) {
   const int64_t N = P->ntarg, NO = P->nptso, NS = P->npts;
-  V2<T>* part = (V2<T>*)(sizeof(T) == 8 ? (void*)P->part64 : (void*)P->part32);
   const V2<T>* wn = (const V2<T>*)(sizeof(T) == 8 ? (void*)P->wnear64 : (void*)P->wnear32);
   int64_t nsplit = 0;
   if (N == 0) return 0;
   if (NO > 0) {
     int64_t wg;
     const int grp = far_group<T>(P, nrhs, prec, &wg);
-    const int rc = dispatch_far<T>(P, mask, grp, wg, q, m, NO, nrhs, part, &nsplit, s);
+    const int rc = dispatch_far<T>(P, mask, grp, wg, q, m, NO, nrhs, part, part_cap, &nsplit, s);
     if (rc) return rc;
   }
   if (ev_far) ACHK(hipEventRecord(ev_far, s));
@@ -846,15 +849,17 @@ static int far_and_csr(ApplyPlan* P, int prec, bool mask, const V2<T>* q, const 
   return 0;
 }
 
+// Work buffers of one apply in the precision of the call: charge and dipole strengths
+// at the oversampled nodes (nptso x nrhs complex each) and the far-sum partials
+// (part_cap complex values, at least ntarg x nrhs: one source split).
+struct ApplyWs {
+  void *q, *m, *part;
+  int64_t part_cap;
+};
+
 // sigma/pot: complex (interleaved) device arrays of length npts * nrhs, leading dim npts
-int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* pot, int nrhs, hipStream_t s,
-                   ApplyTimes* tm) {
-  if (nrhs < 1 || nrhs > 64) return -6;
-  if (prec != 32 && prec != 64) {
-    fprintf(stderr, "fmm3dbie_amd: apply precision %d (must be 32 or 64)\n", prec);
-    return -6;
-  }
-  if (ensure_ws(P, nrhs) != 0) return -2;
+static int apply_run_ws(ApplyPlan* P, int prec, int mode, const void* sigma, void* pot, int nrhs, hipStream_t s,
+                        ApplyTimes* tm, const ApplyWs& W) {
   const int64_t N = P->npts, NT = P->ntarg, NO = P->nptso;
   const double ar = P->alpha[0], ai = P->helm ? P->alpha[1] : 0.0;
   const double br = P->beta[0], bi = P->helm ? P->beta[1] : 0.0;
@@ -869,12 +874,13 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
     if (NO > 0)
       k_oversample<double><<<nblk(NO, 256), 256, 0, s>>>(NO, P->opatch, P->ixyzs, P->ixyzso, P->ioff, P->np_of, P->imat,
                                                        sg, nrhs, N, P->wover, ar, ai, br, bi,
-                                                       (V2<double>*)P->q64, (V2<double>*)P->m64, NO);
+                                                       (V2<double>*)W.q, (V2<double>*)W.m, NO);
     if (tm) ACHK(hipEventRecord(ev[1], s));
     const bool mask = mode == APPLY_MASKED;
-    auto* q = (const V2<double>*)P->q64;
-    auto* m = (const V2<double>*)P->m64;
-    rc = far_and_csr<double>(P, 64, mask, q, m, sg, pt, nrhs, s, tm ? ev[2] : nullptr);
+    auto* q = (const V2<double>*)W.q;
+    auto* m = (const V2<double>*)W.m;
+    rc = far_and_csr<double>(P, 64, mask, q, m, sg, pt, nrhs, (V2<double>*)W.part, W.part_cap, s,
+                             tm ? ev[2] : nullptr);
     if (tm) ACHK(hipEventRecord(ev[3], s));
     if (!rc && !mask) {
       const double kr = P->helm ? P->zk[0] : 0.0, ki = P->helm ? P->zk[1] : 0.0;
@@ -890,11 +896,11 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
     V2<float>* pt = (V2<float>*)pot;
     if (NO > 0)
       k_oversample<float><<<nblk(NO, 256), 256, 0, s>>>(NO, P->opatch, P->ixyzs, P->ixyzso, P->ioff, P->np_of, P->imat,
-                                                      sg, nrhs, N, P->wover32, ar, ai, br, bi, (V2<float>*)P->q32,
-                                                      (V2<float>*)P->m32, NO);
+                                                      sg, nrhs, N, P->wover32, ar, ai, br, bi, (V2<float>*)W.q,
+                                                      (V2<float>*)W.m, NO);
     if (tm) ACHK(hipEventRecord(ev[1], s));
-    rc = far_and_csr<float>(P, 32, true, (const V2<float>*)P->q32, (const V2<float>*)P->m32, sg, pt, nrhs, s,
-                            tm ? ev[2] : nullptr);
+    rc = far_and_csr<float>(P, 32, true, (const V2<float>*)W.q, (const V2<float>*)W.m, sg, pt, nrhs,
+                            (V2<float>*)W.part, W.part_cap, s, tm ? ev[2] : nullptr);
     if (tm) ACHK(hipEventRecord(ev[3], s));
     if (tm) ACHK(hipEventRecord(ev[4], s));
   }
@@ -910,6 +916,84 @@ int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* po
     for (auto& e : ev) (void)hipEventDestroy(e);
   }
   return 0;
+}
+
+static int check_prec_nrhs(int prec, int nrhs) {
+  if (nrhs < 1 || nrhs > 64) return -6;
+  if (prec != 32 && prec != 64) {
+    fprintf(stderr, "fmm3dbie_amd: apply precision %d (must be 32 or 64)\n", prec);
+    return -6;
+  }
+  return 0;
+}
+
+static ApplyWs plan_ws(const ApplyPlan* P, int prec) {
+  const int64_t cap = (int64_t)P->far_split_cap * 64 * P->ntarg;
+  if (prec == 64) return {P->q64, P->m64, P->part64, cap};
+  return {P->q32, P->m32, P->part32, cap};
+}
+
+// sigma/pot: complex (interleaved) device arrays of length npts * nrhs, leading dim npts;
+// work buffers owned by the plan (allocated on the first call that needs them)
+int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* pot, int nrhs, hipStream_t s,
+                   ApplyTimes* tm) {
+  const int rc = check_prec_nrhs(prec, nrhs);
+  if (rc) return rc;
+  if (ensure_ws(P, nrhs) != 0) return -2;
+  return apply_run_ws(P, prec, mode, sigma, pot, nrhs, s, tm, plan_ws(P, prec));
+}
+
+// ------------------------------------------------------------------ allocation-free apply
+// The work buffers of apply_plan_run_noalloc live in this file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never passes `scratch` itself to
+// a kernel (on the host the name is only a stand-in): it uses the device address that
+// hipGetSymbolAddress returns, looked up once per device.
+constexpr size_t APPLY_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[APPLY_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mutex;
+static std::map<int, void*> g_scratch_ptr;  // device -> address of its copy of `scratch`
+
+static void* scratch_address() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    fprintf(stderr, "fmm3dbie_amd: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+    return nullptr;
+  }
+  g_scratch_ptr[dev] = ptr;
+  return ptr;
+}
+
+size_t apply_scratch_bytes() { return APPLY_SCRATCH_BYTES; }
+
+int apply_plan_run_noalloc(ApplyPlan* P, int prec, int mode, const void* sigma, void* pot, int nrhs, hipStream_t s) {
+  int rc = check_prec_nrhs(prec, nrhs);
+  if (rc) return rc;
+  const size_t cs = prec == 64 ? 2 * sizeof(double) : 2 * sizeof(float);  // one complex value
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t strength = up(cs * (size_t)(P->nptso > 0 ? P->nptso : 1) * nrhs);
+  const size_t one_split = cs * (size_t)(P->ntarg > 0 ? P->ntarg : 1) * nrhs;
+  if (2 * strength + one_split > APPLY_SCRATCH_BYTES) {
+    fprintf(stderr,
+            "fmm3dbie_amd: allocation-free apply: %zu bytes of work buffers needed (nptso %lld, targets %lld, "
+            "nrhs %d, prec %d), the fixed scratch holds %zu\n",
+            2 * strength + one_split, (long long)P->nptso, (long long)P->ntarg, nrhs, prec, APPLY_SCRATCH_BYTES);
+    return -7;
+  }
+  unsigned char* base = (unsigned char*)scratch_address();
+  if (!base) return -1;
+  ApplyWs W;
+  W.q = base;
+  W.m = base + strength;
+  W.part = base + 2 * strength;
+  W.part_cap = (int64_t)((APPLY_SCRATCH_BYTES - 2 * strength) / cs);
+  return apply_run_ws(P, prec, mode, sigma, pot, nrhs, s, nullptr, W);
 }
 
 // Autotuner of the far-sum launch shape for (prec, nrhs): every right-hand-side group
@@ -931,10 +1015,11 @@ int apply_plan_tune(ApplyPlan* P, int prec, int nrhs, hipStream_t s, double* rep
   }
   auto far = [&](int gsz, int64_t wg) {
     int64_t nsplit;
+    const int64_t cap = (int64_t)P->far_split_cap * 64 * P->ntarg;
     return prec == 64 ? dispatch_far<double>(P, true, gsz, wg, (const V2<double>*)P->q64, (const V2<double>*)P->m64,
-                                             NO, nrhs, (V2<double>*)P->part64, &nsplit, s)
+                                             NO, nrhs, (V2<double>*)P->part64, cap, &nsplit, s)
                       : dispatch_far<float>(P, true, gsz, wg, (const V2<float>*)P->q32, (const V2<float>*)P->m32, NO,
-                                            nrhs, (V2<float>*)P->part32, &nsplit, s);
+                                            nrhs, (V2<float>*)P->part32, cap, &nsplit, s);
   };
   hipEvent_t e0, e1;
   ACHK(hipEventCreate(&e0));

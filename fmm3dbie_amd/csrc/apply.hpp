@@ -69,6 +69,10 @@ int apply_plan_build(ApplyPlan* P, const ApplyPlanInput& in, hipStream_t s);
 int apply_plan_run(ApplyPlan* P, int prec, int mode, const void* sigma, void* pot, int nrhs, hipStream_t s,
                    ApplyTimes* tm);
 int apply_plan_run_real(ApplyPlan* P, int mode, const double* sigma, double* pot, hipStream_t s);
+// apply_plan_run with its work buffers in a fixed per-device scratch array instead of
+// plan-owned allocations: calls no allocator; -7 when the buffers do not fit
+int apply_plan_run_noalloc(ApplyPlan* P, int prec, int mode, const void* sigma, void* pot, int nrhs, hipStream_t s);
+size_t apply_scratch_bytes();
 void apply_plan_free_ws(ApplyPlan* P);
 // time the far-sum launch shapes for (prec, nrhs) and keep the fastest; report gets
 // (group, workgroup target, ms) triples; returns the number of shapes timed

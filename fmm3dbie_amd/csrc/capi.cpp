@@ -612,6 +612,18 @@ int fmm3dbie_apply_plan_apply(void* plan, int prec, int mode, const void* sigma,
   return rc;
 }
 
+// apply_plan_apply without allocation: work buffers in the library's fixed per-device
+// scratch array (size from fmm3dbie_apply_scratch_bytes); -7 if they do not fit.
+int fmm3dbie_apply_plan_apply_noalloc(void* plan, int prec, int mode, const void* sigma, void* pot, int nrhs,
+                                      void* stream) {
+  if (!plan) return -12;
+  const int rc = apply_plan_run_noalloc((ApplyPlan*)plan, prec, mode, sigma, pot, nrhs, (hipStream_t)stream);
+  g_last_error = rc;
+  return rc;
+}
+
+size_t fmm3dbie_apply_scratch_bytes(void) { return apply_scratch_bytes(); }
+
 int fmm3dbie_apply_plan_info(void* plan, int64_t* out4) {
   if (!plan) return -12;
   ApplyPlan* P = (ApplyPlan*)plan;

@@ -437,6 +437,23 @@ class ApplyPlan:
             times[:] = tm
         return pot
 
+    def apply_noalloc(self, sigma, pot, prec=64, mode=0, stream=None):
+        """The apply with no device allocation (fmm3dbie_apply_plan_apply_noalloc): work
+        buffers in the library's fixed per-device scratch array; pot must be preallocated
+        like sigma.  Raises RuntimeError when the work buffers exceed that array."""
+        torch = _torch()
+        if mode not in (0, 1):
+            raise ValueError("mode must be 0 (add/subtract) or 1 (masked)")
+        nrhs = _dens(sigma, self.npts, prec, "sigma")
+        if _dens(pot, self.npts, prec, "pot") != nrhs:
+            raise ValueError("pot must have the shape of sigma")
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib().fmm3dbie_apply_plan_apply_noalloc(self.handle, prec, mode, _dptr(sigma), _dptr(pot), nrhs,
+                                                     C.c_void_p(s.cuda_stream))
+        if rc != 0:
+            raise RuntimeError("allocation-free apply failed (status %d)" % rc)
+        return pot
+
     def info(self):
         """(npts, nptso, nnz, uniform patch size or 0)."""
         out = np.zeros(4, np.int64)

@@ -15,6 +15,7 @@
  */
 #ifndef FMM3DBIE_AMD_H
 #define FMM3DBIE_AMD_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -290,6 +291,19 @@ void *fmm3dbie_apply_plan_create(int helm, int64_t npatches, const int64_t *nord
  * length npts.  times (optional, 4 doubles): oversample, far, csr, subtract ms. */
 int fmm3dbie_apply_plan_apply(void *plan, int prec, int mode, const void *sigma, void *pot, int nrhs,
                               void *stream, double *times);
+/* No reference counterpart: fmm3dbie_apply_plan_apply for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc/hipFree or allocating library routine: the strength
+ * and partial-sum buffers live in one fixed per-device array of
+ * fmm3dbie_apply_scratch_bytes() bytes (64 MiB), held by each GPU the library runs on
+ * until the process ends.  It needs 2 x nptso x nrhs + ntarg x nrhs complex values (ntarg
+ * = the plan's targets; 16 B each at prec 64, 8 B at prec 32) and returns -7 when they do not fit.  Every call on
+ * one GPU shares that array, so calls that overlap on one GPU (different streams or
+ * threads) race; calls in one stream are safe.  The plan itself is built by the
+ * allocating fmm3dbie_apply_plan_create.  No timing output. */
+int fmm3dbie_apply_plan_apply_noalloc(void *plan, int prec, int mode, const void *sigma, void *pot, int nrhs,
+                                      void *stream);
+size_t fmm3dbie_apply_scratch_bytes(void);
 int fmm3dbie_apply_plan_info(void *plan, int64_t *out4);
 void fmm3dbie_apply_plan_destroy(void *plan);
 /* Autotuner of the all-pairs far-sum launch (no reference counterpart): times every
