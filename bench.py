@@ -1,10 +1,14 @@
 """Benchmark: near-quadrature pairs/s (getnearquad_helm_comb_dir on the GPU) and GMRES
-solve time to 1e-6 (plain FP64 vs FP32-inner + FP64 iterative refinement).
+solve time to 1e-6 relative error (plain FP64 vs learned order selection + FP32-inner /
+FP64-refined GMRES + batched right-hand sides).
 
-Workload (default): Helmholtz combined field, zk=1.1, alpha=i*zk, beta=1, on the
-cube-projected unit sphere with a 12x12 grid per face (1728 triangles, order 4, 25920
-points), eps=1e-6.  A step is one full near-quadrature generation over every near
-(target, patch) pair with all inputs resident in HBM.
+Workload (default, the largest configuration): Helmholtz combined field, zk = 10,
+alpha = i zk, beta = 1, on a sphere deformed radially by 1 + 0.2 sin(3 theta) cos(2 phi),
+cube-projected at 32x32 cells per face (12288 triangles, order 6, 344064 points),
+eps = 1e-6.  A step is one full near-quadrature generation over every near (target,
+patch) pair with all inputs resident in HBM.  The same JSON line carries the combined
+solve of that configuration (16 plane waves) and the solves of the smaller
+configurations (Laplace sphere, Helmholtz sphere, 64 batched plane waves, wavy torus).
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -13,6 +17,10 @@ import json
 import os
 import sys
 import time
+
+# the CPU baseline's OpenMP threads stay next to their cores (read when the runtime starts)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 import numpy as np
 
@@ -81,12 +89,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--config", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-learned", action="store_true", help="skip the torus learned+mixed solve")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-plane-wave batched solve")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the configuration's own sections (no solves of the smaller configurations)")
+    ap.add_argument("--combined-fp64-rhs", type=int, default=2,
+                    help="right-hand sides of the 344k-point combined run solved by plain FP64 GMRES (timed, "
+                         "scaled to 16) and by the converged FP64 comparator")
     ap.add_argument("--batched-sequential", type=int, default=8,
                     help="FP64 single-RHS solves timed for the batched comparison")
     ap.add_argument("--apply-only", action="store_true",
@@ -116,16 +129,20 @@ def main():
     dev = torch.device("cuda:0")
     g, helm, pars, wname, zk = workload(args.config)
     eps = 1e-6
-    t0 = time.time()
-    P = problem.setup(g, eps, helm, pars)
-    t_setup_host = time.time() - t0
-    # the same setup with the near list and far orders computed on the GPU
+    # set-up as the reference's solver does it (near list, quadrature offsets, far orders,
+    # oversampled geometry), the near list and far orders on the GPU; on the smaller
+    # surfaces the host set-up is timed beside it and must give the same lists and orders
     problem.setup(g, eps, helm, pars, gpu_near=True)
     t0 = time.time()
-    Pg = problem.setup(g, eps, helm, pars, gpu_near=True)
+    P = problem.setup(g, eps, helm, pars, gpu_near=True)
     t_setup_gpu = time.time() - t0
-    setup_same = bool(np.array_equal(Pg["col_ind"], P["col_ind"]) and np.array_equal(Pg["novers"], P["novers"]))
-    del Pg
+    t_setup_host, setup_same = None, None
+    if args.config != 4:
+        t0 = time.time()
+        Ph = problem.setup(g, eps, helm, pars)
+        t_setup_host = time.time() - t0
+        setup_same = bool(np.array_equal(Ph["col_ind"], P["col_ind"]) and np.array_equal(Ph["novers"], P["novers"]))
+        del Ph
     N = len(g["srcvals"])
     p = int(np.diff(g["ixyzs"])[0])
     nq = layer.NearQuadDevice(g, helm, pars, eps, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], device=dev)
@@ -200,7 +217,8 @@ def main():
         "kernels_note": "timed steps: fixed-rule and self kernels on a second stream beside the adaptive one; "
                         "kernels_ms_alone: each kernel alone, in order on one stream",
         "kernels_ms_alone": iso,
-        "roofline": {"bound": "mfma", "pipe": "fp64 (VALU; MI355X FP64 vector peak = matrix peak)",
+        "roofline": {"bound": "valu", "pipe": "fp64 VALU (no MFMA in this kernel; MI355X FP64 vector peak = "
+                                              "FP64 matrix peak = 78.6 TFLOP/s)",
                      "kernel": "k_adaptive", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
                      "alone": {"ms": iso["adaptive"], "achieved": fl / (iso["adaptive"] * 1e-3) / 1e12,
@@ -225,84 +243,102 @@ def main():
         tg = time.perf_counter() - t0
         out["findnear"] = {"gpu_s_incl_copies": tg, "same_as_setup": bool(np.array_equal(rpg, P["row_ptr"])
                                                                             and np.array_equal(cig, P["col_ind"]))}
-    elif not args.no_solve and helm:
-        x0 = np.array([0.1, 0.2, -0.3])
-        rhs = torch.as_tensor(problem.point_source(g["srcvals"][:, :3], x0, zk), device=dev)
-        wts = torch.as_tensor(P["wts"], device=dev)
-        plan = layer.ApplyPlan(g, helm, pars, P["row_ptr"], P["col_ind"], P["iquad"], nq.wnear, P["novers"],
-                               P["ixyzso"], P["srcover"], P["whtsover"])
-        zid = 0.5
-        npts_, nptso, _, _ = plan.info()
-        tm = np.zeros(4)
-        s = rhs.clone()
-        s32 = s.to(torch.complex64)
-        plan.apply(s, 64, 0)
-        plan.apply(s32, 32, 1)
-        t64 = np.zeros(4)
-        t32 = np.zeros(4)
-        for _ in range(3):  # per-phase HIP-event times, mean of 3 warm applies each
-            plan.apply(s, 64, 0, times=tm)
-            t64 += tm / 3
-            plan.apply(s32, 32, 1, times=tm)
-            t32 += tm / 3
-        # warm
-        layer.gmres(plan, rhs, zid, wts, numit=200, eps_gmres=1e-6)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        x64, nit, errs, rres64, nap = layer.gmres(plan, rhs, zid, wts, numit=200, eps_gmres=1e-6)
-        torch.cuda.synchronize()
-        ts64 = time.perf_counter() - t0
-        layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=5e-7, tol=1e-6)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        xir, outer, inner, rresir, n32, n64 = layer.gmres_ir(plan, rhs, zid, wts, numit=200, eps_inner=5e-7, tol=1e-6)
-        torch.cuda.synchronize()
-        tsir = time.perf_counter() - t0
-        # exterior field error at 10 points on radius 3 vs the exact point-source field
-        dirs = problem.plane_wave_dirs(10, 0)
-        targ = 3.0 * dirs
-        exact = problem.point_source(targ, x0, zk)
-
-        def field(x):
-            xs = x.cpu().numpy()
-            if xs.ndim > 1:
-                xs = xs[0]
-            d = targ[:, None, :] - g["srcvals"][None, :, :3]
-            r = np.linalg.norm(d, axis=-1)
-            dn = (d * g["srcvals"][None, :, 9:12]).sum(-1)
-            ker = (pars[2] * dn * (1 - 1j * zk * r) / r ** 3 + pars[1] / r) * np.exp(1j * zk * r) / (4 * np.pi)
-            return ker @ (xs * P["wts"])
-
-        e64 = float(np.abs(field(x64) - exact).max() / np.abs(exact).max())
-        eir = float(np.abs(field(xir) - exact).max() / np.abs(exact).max())
-        xd = (xir - x64).cpu().numpy()
-        rel_sol = float(np.sqrt((np.abs(xd) ** 2 * P["wts"]).sum() / (np.abs(x64.cpu().numpy()) ** 2 * P["wts"]).sum()))
-        out["solve"] = {"gmres_fp64_s": ts64, "gmres_fp64_iters": int(nit[0]), "gmres_fp64_rres": float(rres64[0]),
-                        "ir_mixed_s": tsir, "ir_outer": outer, "ir_inner_iters": inner,
-                        "ir_fp32_applies": n32, "ir_fp64_applies": n64, "ir_rres": float(rresir[0]),
-                        "speedup_mixed_vs_fp64": ts64 / tsir, "field_err_fp64": e64, "field_err_mixed": eir,
-                        "mixed_vs_fp64_solution_rel": rel_sol,
-                        "nptso": nptso,
-                        "far_fp64_tflops": npts_ * nptso * far_flops_per_pair(1) / (t64[1] * 1e-3) / 1e12,
-                        "far_fp32_tflops": npts_ * nptso * far_flops_per_pair(1) / (t32[1] * 1e-3) / 1e12,
-                        # near correction (fused with the far reduction): wnear + CSR index arrays +
-                        # row pointers + sigma and pot (the partial sums read are not counted)
-                        "csr_fp64_gbs": (16.0 * P["nquad"] + 16.0 * P["nnz"] + 8.0 * (N + 1) + 32.0 * N)
-                        / (t64[2] * 1e-3) / 1e9,
-                        "csr_fp32_gbs": (8.0 * P["nquad"] + 16.0 * P["nnz"] + 8.0 * (N + 1) + 16.0 * N)
-                        / (t32[2] * 1e-3) / 1e9,
-                        "apply_fp64_ms": {"oversample": t64[0], "far": t64[1], "csr": t64[2], "subtract": t64[3]},
-                        "apply_fp32_ms": {"oversample": t32[0], "far": t32[1], "csr": t32[2]}}
-        plan.close()
+        if not args.no_solve:
+            out["combined_solve"] = combined_solve(g, pars, P, nq, dev, dt, args.combined_fp64_rhs)
+    elif not args.no_solve and helm and args.config == 1:
+        out["solve"] = sphere_solve(dev, (g, pars, P, nq.wnear))
+    del nq
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, helm, pars, eps, P, args.cpu_seconds)
-    if args.config == 1 and not args.no_solve:
+    if not args.no_solve and not args.no_extra:
         out["laplace_solve"] = laplace_solve(dev)
+        if args.config != 1:
+            out["sphere_solve"] = sphere_solve(dev)
         if not args.no_learned:
             out["learned_mixed_solve"] = learned_mixed_solve(dev)
         if not args.no_batched:
             out["batched_solve"] = batched_solve(dev, args.batched_sequential)
     print(json.dumps(out))
+
+
+# Residual tolerance of the timed solves.  With the combined-field operator's condition
+# number a relative residual of 3e-7 keeps every solution within 1e-6 (weighted relative
+# l2) of the converged FP64 solution, which each section reports.
+RTOL = 3e-7
+RTOL_REF = 1e-10  # the converged FP64 comparator (untimed)
+
+
+def _wrel(x, ref, wts):
+    """Relative error in the quadrature-weighted l2 norm (max over right-hand sides)."""
+    x = np.atleast_2d(x)
+    ref = np.atleast_2d(ref)
+    return float(max(np.sqrt((np.abs(a - b) ** 2 * wts).sum() / (np.abs(b) ** 2 * wts).sum()) for a, b in zip(x, ref)))
+
+
+def sphere_solve(dev, case=None):
+    """Helmholtz zk = 1.1, alpha = i zk, beta = 1, cube sphere 12x12 cells per face, order 4
+    (25920 points), point source at (0.1, 0.2, -0.3): plain FP64 GMRES against FP32 inner
+    GMRES with FP64 iterative refinement, both to a relative residual of RTOL; each
+    solution's error against a converged FP64 solve, the exterior field at 10 points of
+    radius 3 against the exact field, and the per-phase apply times."""
+    import torch
+    from fmm3dbie_amd import geometry as G, layer, problem
+    if case is None:
+        g = G.get_sphere_npat(1.0, 12, (0, 0, 0), 4)
+        pars = np.array([1.1, 1.1j, 1.0])
+        P = problem.setup(g, 1e-6, True, pars, gpu_near=True)
+        nq = layer.NearQuadDevice(g, True, pars, 1e-6, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], device=dev)
+        wnear = nq.run()
+    else:
+        g, pars, P, wnear = case
+    zk = float(np.real(pars[0]))
+    N = len(g["srcvals"])
+    x0 = np.array([0.1, 0.2, -0.3])
+    rhs = torch.as_tensor(problem.point_source(g["srcvals"][:, :3], x0, zk), device=dev)
+    wts = torch.as_tensor(P["wts"], device=dev)
+    plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], wnear, P["novers"],
+                           P["ixyzso"], P["srcover"], P["whtsover"])
+    npts_, nptso, _, _ = plan.info()
+    tm, t64, t32 = np.zeros(4), np.zeros(4), np.zeros(4)
+    s32 = rhs.to(torch.complex64)
+    plan.apply(rhs, 64, 0)
+    plan.apply(s32, 32, 1)
+    for _ in range(3):  # per-phase HIP-event times, mean of 3 warm applies each
+        plan.apply(rhs, 64, 0, times=tm)
+        t64 += tm / 3
+        plan.apply(s32, 32, 1, times=tm)
+        t32 += tm / 3
+    xref, nref_it, _, rref, _ = layer.gmres(plan, rhs, 0.5, wts, numit=200, eps_gmres=RTOL_REF)
+    layer.gmres(plan, rhs, 0.5, wts, numit=200, eps_gmres=RTOL)
+    (x64, nit, _, rres64, _), ts64 = _timed(lambda: layer.gmres(plan, rhs, 0.5, wts, numit=200, eps_gmres=RTOL))
+    layer.gmres_ir(plan, rhs, 0.5, wts, numit=200, eps_inner=RTOL / 2, tol=RTOL)
+    (xir, outer, inner, rresir, n32, n64), tsir = _timed(
+        lambda: layer.gmres_ir(plan, rhs, 0.5, wts, numit=200, eps_inner=RTOL / 2, tol=RTOL))
+    targ = 3.0 * problem.plane_wave_dirs(10, 0)
+    exact = problem.point_source(targ, x0, zk)
+    xr = xref.cpu().numpy()
+    f64 = _exterior_field(g, P["wts"], x64.cpu().numpy(), targ, pars, zk)[0]
+    fir = _exterior_field(g, P["wts"], xir.cpu().numpy(), targ, pars, zk)[0]
+    res = {"workload": "helmholtz_zk1.1_sphere_12x12_o4_pointsource", "npts": N, "nnz": P["nnz"],
+           "rtol": RTOL, "gmres_fp64_s": ts64, "gmres_fp64_iters": int(nit[0]), "gmres_fp64_rres": float(rres64[0]),
+           "ir_mixed_s": tsir, "ir_outer": outer, "ir_inner_iters": inner, "ir_fp32_applies": n32,
+           "ir_fp64_applies": n64, "ir_rres": float(rresir[0]), "speedup_mixed_vs_fp64": ts64 / tsir,
+           "reference_solve": {"rres": float(rref[0]), "iters": int(nref_it[0])},
+           "err_vs_converged_fp64": {"plain_fp64": _wrel(x64.cpu().numpy(), xr, P["wts"]),
+                                     "mixed": _wrel(xir.cpu().numpy(), xr, P["wts"])},
+           "field_err_fp64": float(np.abs(f64 - exact).max() / np.abs(exact).max()),
+           "field_err_mixed": float(np.abs(fir - exact).max() / np.abs(exact).max()),
+           "nptso": nptso,
+           "far_fp64_tflops": npts_ * nptso * far_flops_per_pair(1) / (t64[1] * 1e-3) / 1e12,
+           "far_fp32_tflops": npts_ * nptso * far_flops_per_pair(1) / (t32[1] * 1e-3) / 1e12,
+           # near correction (fused with the far reduction): wnear + CSR index arrays + row
+           # pointers + sigma and pot (the partial sums read are not counted)
+           "csr_fp64_gbs": (16.0 * P["nquad"] + 16.0 * P["nnz"] + 8.0 * (N + 1) + 32.0 * N) / (t64[2] * 1e-3) / 1e9,
+           "csr_fp32_gbs": (8.0 * P["nquad"] + 16.0 * P["nnz"] + 8.0 * (N + 1) + 16.0 * N) / (t32[2] * 1e-3) / 1e9,
+           "apply_fp64_ms": {"oversample": t64[0], "far": t64[1], "csr": t64[2], "subtract": t64[3]},
+           "apply_fp32_ms": {"oversample": t32[0], "far": t32[1], "csr": t32[2]}}
+    plan.close()
+    return res
 
 
 def _exterior_field(g, wts, x, targ, pars, zk):
@@ -425,32 +461,43 @@ def learned_mixed_solve(dev):
     wts = torch.as_tensor(P["wts"], device=dev)
     nq = layer.NearQuadDevice(g, True, pars, 1e-6, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], device=dev)
     # labels and features from one plain run, GPU training on 2e5 sampled pairs (seed 3)
-    X, y, idx = S.collect(nq)
+    (X, y, idx, linfo), tcol = _timed(lambda: S.collect(nq))
     params, info = S.train(X, y, n_train=200000, epochs=30, seed=3, device=dev)
+    lmax = S.level_caps(y, linfo)
 
     def plain():
         w = nq.run()
         plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"], P["ixyzso"],
                                P["srcover"], P["whtsover"])
         t0 = time.perf_counter()
-        x, nit, _, rres, _ = layer.gmres(plan, rhs, 0.5, wts, numit=300, eps_gmres=1e-6)
+        x, nit, _, rres, _ = layer.gmres(plan, rhs, 0.5, wts, numit=300, eps_gmres=RTOL)
         torch.cuda.synchronize()
         ts = time.perf_counter() - t0
         plan.close()
         return x, int(nit[0]), float(rres[0]), ts
 
     def learned():
-        w = nq.run(selector=params)
+        w = nq.run(selector=params, route_lmax=lmax)
         plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"], P["ixyzso"],
                                P["srcover"], P["whtsover"])
         # one right-hand side: the default far-sum shape (tuning costs more than it saves
         # in a single solve)
         t0 = time.perf_counter()
-        x, outer, inner, rr, n32, n64 = layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=5e-7, tol=1e-6)
+        x, outer, inner, rr, n32, n64 = layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=RTOL / 2,
+                                                       tol=RTOL)
         torch.cuda.synchronize()
         ts = time.perf_counter() - t0
         plan.close()
         return x, outer, inner, float(rr[0]), ts
+
+    def converged():
+        """FP64 solve of the plain operator to RTOL_REF (untimed comparator)."""
+        w = nq.run()
+        plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"], P["ixyzso"],
+                               P["srcover"], P["whtsover"])
+        x, nit, _, rres, _ = layer.gmres(plan, rhs, 0.5, wts, numit=300, eps_gmres=RTOL_REF)
+        plan.close()
+        return x.cpu().numpy(), float(rres[0]), int(nit[0])
 
     plain()
     (x64, nit, rr64, ts64), tt64 = _timed(plain)
@@ -460,9 +507,9 @@ def learned_mixed_solve(dev):
     layer.set_side_stream(False)
     nq.run()
     q64 = {k: nq.stats[k] for k in ("ms_tree", "ms_adap", "ms_fixed", "ms_self")}
-    nq.run(selector=params)
-    ql = {k: nq.stats[k] for k in ("ms_select", "ms_tree", "ms_shallow", "ms_adap", "ms_fixed", "ms_self",
-                                   "nshallow0", "nshallow1", "nfallback")}
+    nq.run(selector=params, route_lmax=lmax)
+    ql = {k: nq.stats[k] for k in ("ms_select", "ms_tree", "ms_tree_routed", "ms_adap", "ms_fixed", "ms_self",
+                                   "nfallback", "ncls")}
     layer.set_side_stream(True)
     targ = np.array([[4.0, 0.0, 1.0], [0.0, 4.0, -1.0], [-3.5, 1.0, 2.0], [0.5, -3.8, 0.5], [0.0, 0.0, 3.0],
                      [2.0, 2.0, 2.5], [-2.0, 2.5, -2.0], [3.0, -3.0, 0.0], [0.0, 0.0, -3.5], [-4.0, -1.0, 0.0]])
@@ -471,16 +518,18 @@ def learned_mixed_solve(dev):
                 / np.abs(exact).max())
     el = float(np.abs(_exterior_field(g, P["wts"], xl.cpu().numpy(), targ, pars, zk) - exact).max()
                / np.abs(exact).max())
-    xd = (xl - x64).cpu().numpy()
-    rel = float(np.sqrt((np.abs(xd) ** 2 * P["wts"]).sum() / (np.abs(x64.cpu().numpy()) ** 2 * P["wts"]).sum()))
+    xc, rrc, itc = converged()
     return {"workload": "helmholtz_zk3_wavytorus_48x16_o6_pointsource", "npts": len(g["srcvals"]), "nnz": P["nnz"],
+            "rtol": RTOL, "reference_solve": {"rres": rrc, "iters": itc},
             "plain_fp64": {"setup_plus_solve_s": tt64, "solve_s": ts64, "iters": nit, "rres": rr64,
-                           "quad_ms": q64, "field_err": e64},
+                           "quad_ms": q64, "field_err": e64,
+                           "err_vs_converged_fp64": _wrel(x64.cpu().numpy(), xc, P["wts"])},
             "learned_mixed": {"setup_plus_solve_s": ttl, "solve_s": tsl, "ir_outer": outer, "ir_inner_iters": inner,
-                              "rres": rrl, "quad_ms": ql, "field_err": el, "solution_rel_vs_fp64": rel},
+                              "rres": rrl, "quad_ms": ql, "field_err": el,
+                              "err_vs_converged_fp64": _wrel(xl.cpu().numpy(), xc, P["wts"])},
             "speedup_solve": ts64 / tsl, "speedup_setup_plus_solve": tt64 / ttl,
             "adaptive_part_ms": {"plain": q64["ms_tree"] + q64["ms_adap"],
-                                 "learned": ql["ms_select"] + ql["ms_tree"] + ql["ms_shallow"] + ql["ms_adap"]},
+                                 "learned": ql["ms_select"] + ql["ms_tree"] + ql["ms_tree_routed"] + ql["ms_adap"]},
             "selector": {"train_pairs": info["n_train"], "heldout_pairs": info["n_heldout"],
                          "acc_train": info["acc_train"], "acc_heldout": info["acc_heldout"],
                          "train_ms": info["ms_train"], "loss_first_last": [info["loss"][0], info["loss"][-1]]}}
@@ -504,20 +553,21 @@ def batched_solve(dev, nseq):
     w = nq.run()
     plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"], P["ixyzso"],
                            P["srcover"], P["whtsover"])
-    layer.gmres(plan, rhs[0], 0.5, wts, numit=300, eps_gmres=1e-6)
+    layer.gmres(plan, rhs[0], 0.5, wts, numit=300, eps_gmres=RTOL)
     xs, iters = [], []
-    _, tseq = _timed(lambda: [xs.append(layer.gmres(plan, rhs[r], 0.5, wts, numit=300, eps_gmres=1e-6))
+    _, tseq = _timed(lambda: [xs.append(layer.gmres(plan, rhs[r], 0.5, wts, numit=300, eps_gmres=RTOL))
                               for r in range(nseq)])
     iters = [int(x[1][0]) for x in xs]
     x64 = torch.stack([x[0] for x in xs]).cpu().numpy()
     tune, ttune = _timed(lambda: plan.tune(32, 64))
     best = min(tune, key=lambda t: t[2])
-    layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=5e-7, tol=1e-6)
+    layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=RTOL / 2, tol=RTOL)
     (xb, outer, inner, rr, n32, n64), tb = _timed(
-        lambda: layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=5e-7, tol=1e-6))
+        lambda: layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=RTOL / 2, tol=RTOL))
     xb = xb.cpu().numpy()
-    rel = max(float(np.sqrt((np.abs(xb[r] - x64[r]) ** 2 * P["wts"]).sum() / (np.abs(x64[r]) ** 2 * P["wts"]).sum()))
-              for r in range(nseq))
+    # converged FP64 comparator on the sequentially solved right-hand sides (untimed)
+    xc, _, _, rrc, _ = layer.gmres(plan, rhs[:nseq].contiguous(), 0.5, wts, numit=300, eps_gmres=RTOL_REF)
+    xc = xc.cpu().numpy()
     targ = 3.0 * problem.plane_wave_dirs(10, 0)
     exact = problem.mie_sound_soft(targ, dirs[:8], zk)
     u = _exterior_field(g, P["wts"], xb[:8], targ, pars, zk)
@@ -529,24 +579,51 @@ def batched_solve(dev, nseq):
             "fp64_sequential_s_per_rhs": tseq / nseq, "fp64_sequential_s_64_scaled": tseq / nseq * 64,
             "fp64_sequential_timed_rhs": nseq, "fp64_iters": iters,
             "batched_mixed_s": tb, "ir_outer": outer, "ir_inner_iters": inner, "ir_rres_max": float(np.max(rr)),
-            "speedup_vs_fp64_sequential": tseq / nseq * 64 / tb, "solution_rel_vs_fp64_max": rel,
+            "speedup_vs_fp64_sequential": tseq / nseq * 64 / tb, "rtol": RTOL,
+            "reference_solve": {"rhs": nseq, "rres_max": float(np.max(rrc))},
+            "err_vs_converged_fp64_max": {"plain_fp64": _wrel(x64, xc, P["wts"]),
+                                          "batched_mixed": _wrel(xb[:nseq], xc, P["wts"])},
             "mie_field_err_8rhs": emie, "tuned_far": {"group": best[0], "wg_target": best[1], "ms": best[2]},
             "tune_s": ttune, "apply_fp32_64rhs_ms": {"oversample": tm[0], "far": tm[1], "reduce_csr": tm[2]}}
 
 
+def _host_cores():
+    """(cores usable by this process, nproc, CPU model): the affinity mask, capped by a
+    cgroup CPU quota when one is set (cpu.max)."""
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    usable = nproc
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            usable = max(1, min(nproc, int(int(q) // int(per))))
+    except (OSError, ValueError):
+        pass
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return usable, nproc, model
+
+
 def cpu_baseline(g, helm, pars, eps, P, budget_s):
-    """The C oracle (FP64, OpenMP) on a bounded sample of target rows of the same
-    workload; rate in near-quad pairs/s."""
+    """The C oracle (FP64 restatement of the reference algorithm, OpenMP over target rows,
+    threads bound close to their cores) on every core this process may use, over a
+    bounded sample of evenly spaced target rows of the same workload; rate in near-quad
+    pairs/s (pairs of the sample / time)."""
     from oracle import oracle as O
-    ncores = min(os.cpu_count() or 1, 16)
-    os.environ["OMP_NUM_THREADS"] = str(ncores)
+    usable, nproc, model = _host_cores()
+    threads = O.set_num_threads(usable)
     N = len(g["srcvals"])
     row_ptr = P["row_ptr"]
     kind = O.HELM_COMB if helm else O.LAP_COMB
-    take = max(64, N // 64)
+    take = max(64, N // 512)
     rate = None
     used = 0
-    for _ in range(6):
+    for _ in range(8):
         rows = np.linspace(0, N - 1, take).astype(np.int64)
         sub_ptr = np.concatenate([[1], 1 + np.cumsum(np.diff(row_ptr)[rows])]).astype(np.int64)
         cols = np.concatenate([P["col_ind"][row_ptr[r] - 1: row_ptr[r + 1] - 1] for r in rows])
@@ -561,10 +638,77 @@ def cpu_baseline(g, helm, pars, eps, P, budget_s):
         if dt > budget_s / 3 or take >= N:
             break
         take = min(N, int(take * max(2.0, budget_s / 2 / max(dt, 1e-3))))
-    return {"value": rate, "unit": "pairs/s", "cores": ncores, "kind": "port",
-            "sample": "%d near pairs from %d evenly spaced target rows of the same workload (C oracle, "
-                      "adaptive self rule replaced by the same polar rule as the GPU)" % (used, take)}
+    return {"value": rate, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "cpu_model": model, "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
+            "sample": "%d near pairs of %d evenly spaced target rows (of %d) of the same workload, C oracle "
+                      "(same self, adaptive and fixed rules as the GPU) on %d threads" % (used, take, N, threads)}
 
+
+def combined_solve(g, pars, P, nq, dev, t_quad, nfp):
+    """The 344k-point configuration (deformed sphere 32x32 cells per face, order 6, zk =
+    10) with 16 plane waves (directions from seed 4, sound-soft data f = -u_inc):
+
+    * plain FP64 GPU form: the plain adaptive quadrature (the benchmark step) and FP64
+      GMRES one right-hand side at a time to a relative residual of RTOL, timed on nfp
+      right-hand sides and scaled to 16;
+    * learned + mixed + batched form: the learned selector trained on this surface's own
+      adaptive pairs (training time reported apart), the routed quadrature, the far-sum
+      launch tuned for 16 right-hand sides, and FP32-inner / FP64-refined GMRES over all
+      16 right-hand sides in lockstep to the same RTOL;
+    * a converged FP64 solve (RTOL_REF, untimed) of the first nfp right-hand sides: the
+      error of both forms against it."""
+    import torch
+    from fmm3dbie_amd import layer, problem, selector as S
+    zk = float(np.real(pars[0]))
+    dirs = problem.plane_wave_dirs(16, 4)
+    rhs = torch.as_tensor(-problem.plane_waves(g["srcvals"][:, :3], dirs, zk), device=dev)
+    wts = torch.as_tensor(P["wts"], device=dev)
+    w_plain = nq.run().clone()
+    plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], w_plain, P["novers"],
+                           P["ixyzso"], P["srcover"], P["whtsover"])
+    plan.apply(rhs[0], 64, 0)
+    xs = []
+    _, tseq = _timed(lambda: [xs.append(layer.gmres(plan, rhs[r], 0.5, wts, numit=300, eps_gmres=RTOL))
+                              for r in range(nfp)])
+    x64 = torch.stack([x[0] for x in xs]).cpu().numpy()
+    it64 = [int(x[1][0]) for x in xs]
+    xc, itc, _, rrc, _ = layer.gmres(plan, rhs[:nfp].contiguous(), 0.5, wts, numit=400, eps_gmres=RTOL_REF)
+    xc = xc.cpu().numpy()
+    plan.close()
+    del w_plain
+    # learned + mixed + batched
+    (X, y, idx, linfo), tcol = _timed(lambda: S.collect(nq))
+    params, info = S.train(X, y, n_train=200000, epochs=30, seed=3, device=dev)
+    lmax = S.level_caps(y, linfo)
+    nq.run(selector=params, route_lmax=lmax)
+    _, tq_l = _timed(lambda: nq.run(selector=params, route_lmax=lmax))
+    st = dict(nq.stats)
+    plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], nq.wnear, P["novers"],
+                           P["ixyzso"], P["srcover"], P["whtsover"])
+    tune, ttune = _timed(lambda: plan.tune(32, 16))
+    plan.apply(rhs.to(torch.complex64), 32, 1)
+    (xb, outer, inner, rr, n32, n64), tb = _timed(
+        lambda: layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=RTOL / 2, tol=RTOL))
+    xb = xb.cpu().numpy()
+    plan.close()
+    plain_total = t_quad + tseq / nfp * 16
+    learned_total = tq_l + ttune + tb
+    return {"workload": "helmholtz_zk10_deformedsphere_32x32_o6_16planewaves", "npts": len(g["srcvals"]),
+            "nrhs": 16, "rtol": RTOL,
+            "plain_fp64": {"quad_s": t_quad, "solve_s_per_rhs": tseq / nfp, "solve_s_16_scaled": tseq / nfp * 16,
+                           "timed_rhs": nfp, "iters": it64, "setup_plus_solve_s": plain_total,
+                           "err_vs_converged_fp64_max": _wrel(x64, xc, P["wts"])},
+            "learned_mixed_batched": {"quad_s": tq_l, "tune_s": ttune, "solve_s": tb, "ir_outer": outer,
+                                      "ir_inner_iters": inner, "ir_fp32_applies": n32, "ir_fp64_applies": n64,
+                                      "rres_max": float(np.max(rr)), "setup_plus_solve_s": learned_total,
+                                      "err_vs_converged_fp64_max": _wrel(xb[:nfp], xc, P["wts"]),
+                                      "quad_stats": {k: st[k] for k in ("ms_select", "ms_tree", "ms_tree_routed",
+                                                                        "ms_adap", "ms_fixed", "ms_self")},
+                                      "selector": {"label_collection_s": tcol, "train_ms": info["ms_train"],
+                                                   "acc_heldout": info["acc_heldout"],
+                                                   "train_pairs": info["n_train"]}},
+            "reference_solve": {"rhs": nfp, "iters": [int(v) for v in itc], "rres_max": float(np.max(rrc))},
+            "speedup_solve": (tseq / nfp * 16) / tb, "speedup_setup_plus_solve": plain_total / learned_total}
 
 if __name__ == "__main__":
     main()

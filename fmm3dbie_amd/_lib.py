@@ -24,7 +24,7 @@ SIGS = {
     "get_rfacs_": ([vp, vp, vp, vp], None),
     "fmm3dbie_last_error": ([], i32),
     "fmm3dbie_nearquad_dev": ([i32, i32, i32, i64, vp, vp, vp, i64, i64, vp, vp, vp, f64, vp, i64, vp, vp, vp,
-                               f64, vp, i32, f64, vp, vp, vp, vp, vp, vp, vp], i32),
+                               f64, vp, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
     "fmm3dbie_mlp_nparam": ([], i32),
     "fmm3dbie_release_workspace": ([], None),
     "fmm3dbie_set_side_stream": ([C.c_int], None),
@@ -43,6 +43,10 @@ SIGS = {
     "fmm3dbie_apply_plan_apply": ([vp, i32, i32, vp, vp, i32, vp, vp], i32),
     "fmm3dbie_apply_plan_apply_noalloc": ([vp, i32, i32, vp, vp, i32, vp], i32),
     "fmm3dbie_apply_scratch_bytes": ([], C.c_size_t),
+    "oversample_geom_": ([vp] * 11, None),
+    "fmm3dbie_host_plan_builds": ([], i64),
+    "get_qwts_": ([vp] * 7, None),
+    "fmm3dbie_oversample_geom_dev": ([i64, i32, vp, vp, vp, vp, vp, vp, vp, vp], i32),
     "fmm3dbie_apply_plan_info": ([vp, vp], i32),
     "fmm3dbie_apply_plan_destroy": ([vp], None),
     "fmm3dbie_apply_plan_tune": ([vp, i32, i32, vp, vp, i32], i32),

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <vector>
+#include <mutex>
 
 #include "apply.hpp"
 #include "gmres.hpp"
@@ -130,6 +131,57 @@ void patch_id_uvs(int64_t npatches, const int64_t* norders, const int64_t* ixyzs
   }
 }
 
+// Apply plans of the host lpcomp drop-ins, kept between calls: a caller such as the
+// reference's zgmres_guru calls lpcomp_*_addsub with the same discretization and weights
+// once per iteration (gmres_routs.f90:241-246), which must not rebuild and re-upload the
+// plan each time.  The key holds every array's address and length, the kernel parameters
+// and a fingerprint of 257 evenly spaced entries of each array, so a call with other data
+// at the same addresses builds a new plan; fmm3dbie_release_workspace() drops it.
+struct HostPlanKey {
+  std::vector<uint64_t> v;
+  bool operator==(const HostPlanKey& o) const { return v == o.v; }
+};
+
+template <typename T>
+void key_array(HostPlanKey& k, const T* a, int64_t n) {
+  k.v.push_back((uint64_t)(uintptr_t)a);
+  k.v.push_back((uint64_t)n);
+  if (!a || n <= 0) return;
+  const int64_t m = n < 257 ? n : 257;
+  for (int64_t i = 0; i < m; i++) {
+    const int64_t j = m > 1 ? (n - 1) * i / (m - 1) : 0;
+    uint64_t b = 0;
+    std::memcpy(&b, a + j, sizeof(T) < 8 ? sizeof(T) : 8);
+    k.v.push_back(b);
+  }
+}
+
+// (plain pointers: nothing is freed by static destruction at process exit, when the HIP
+// runtime may already be gone)
+struct HostPlanCache {
+  HostPlanKey key;
+  ApplyPlan* plan = nullptr;
+  hipStream_t s = nullptr;
+  double *ds = nullptr, *dp = nullptr;
+  size_t ns = 0, nt = 0;
+  int dev = -1;
+  void clear() {
+    delete plan;
+    plan = nullptr;
+    if (ds) (void)hipFree(ds);
+    if (dp) (void)hipFree(dp);
+    if (s) (void)hipStreamDestroy(s);
+    ds = dp = nullptr;
+    s = nullptr;
+    ns = nt = 0;
+    key.v.clear();
+    dev = -1;
+  }
+};
+std::mutex g_hp_mutex;
+HostPlanCache g_hp;
+int64_t g_hp_builds = 0;  // plans built by the host drop-ins (for tests)
+
 int lpcomp_host(bool helm, int64_t npatches, const int64_t* norders, const int64_t* ixyzs, const int64_t* iptype,
                 int64_t npts, const double* srcvals, const double* kpar6, int64_t nnz, const int64_t* row_ptr,
                 const int64_t* col_ind, const int64_t* iquad, int64_t nquad, const double* wnear,
@@ -141,40 +193,72 @@ int lpcomp_host(bool helm, int64_t npatches, const int64_t* norders, const int64
       fprintf(stderr, "fmm3dbie_amd: only triangular RV patches (iptype=1) are supported\n");
       return -7;
     }
-  hipStream_t s;
-  if (hipStreamCreate(&s) != hipSuccess) return -1;
-  ApplyPlanInput in = {};
-  in.helm = helm;
-  in.npatches = npatches; in.npts = npts; in.nptso = nptso; in.nnz = nnz; in.nquad = nquad;
-  in.norders = norders; in.ixyzs = ixyzs; in.novers = novers; in.ixyzso = ixyzso;
-  in.row_ptr = row_ptr; in.col_ind = col_ind; in.iquad = iquad;
-  in.srcvals = srcvals; in.srcover = srcover; in.whtsover = whtsover;
-  in.wnear = wnear; in.wnear_is_device = false;
-  in.zk[0] = kpar6[0]; in.zk[1] = kpar6[1];
-  in.alpha[0] = kpar6[2]; in.alpha[1] = kpar6[3];
-  in.beta[0] = kpar6[4]; in.beta[1] = kpar6[5];
-  in.targs = targs;
-  in.ndtarg = ndtarg;
-  in.ntarg = targs ? ntarg : npts;
-  int rc = 0;
-  {
-    ApplyPlan P;
-    rc = apply_plan_build(&P, in, s);
-    const size_t n = (helm ? 2 : 1) * (size_t)npts, nt = (helm ? 2 : 1) * (size_t)in.ntarg;
-    double *ds = nullptr, *dp = nullptr;
-    if (rc == 0 && (hipMalloc((void**)&ds, (n ? n : 1) * 8) != hipSuccess ||
-                    hipMalloc((void**)&dp, (nt ? nt : 1) * 8) != hipSuccess))
-      rc = -2;
-    if (rc == 0 && n && hipMemcpyAsync(ds, sigma, n * 8, hipMemcpyHostToDevice, s) != hipSuccess) rc = -1;
-    if (rc == 0)
-      rc = helm ? apply_plan_run(&P, 64, APPLY_ADDSUB, ds, dp, 1, s, nullptr)
-                : apply_plan_run_real(&P, APPLY_ADDSUB, ds, dp, s);
-    if (rc == 0 && nt && hipMemcpyAsync(pot, dp, nt * 8, hipMemcpyDeviceToHost, s) != hipSuccess) rc = -1;
-    if (hipStreamSynchronize(s) != hipSuccess && rc == 0) rc = -1;
-    if (ds) (void)hipFree(ds);
-    if (dp) (void)hipFree(dp);
+  const int64_t nt_eff = targs ? ntarg : npts;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  HostPlanKey key;
+  key.v = {(uint64_t)helm, (uint64_t)npatches, (uint64_t)npts, (uint64_t)nnz, (uint64_t)nquad, (uint64_t)nptso,
+           (uint64_t)ndtarg, (uint64_t)nt_eff, (uint64_t)dev};
+  for (int i = 0; i < 6; i++) {  // the kernel parameters by value (the caller's copy may move)
+    uint64_t b = 0;
+    std::memcpy(&b, kpar6 + i, 8);
+    key.v.push_back(b);
   }
-  (void)hipStreamDestroy(s);
+  key_array(key, norders, npatches);
+  key_array(key, ixyzs, npatches + 1);
+  key_array(key, srcvals, 12 * npts);
+  key_array(key, row_ptr, nt_eff + 1);
+  key_array(key, col_ind, nnz);
+  key_array(key, iquad, nnz + 1);
+  key_array(key, wnear, (helm ? 2 : 1) * nquad);
+  key_array(key, novers, npatches);
+  key_array(key, ixyzso, npatches + 1);
+  key_array(key, srcover, 12 * nptso);
+  key_array(key, whtsover, nptso);
+  key_array(key, targs, targs ? ndtarg * ntarg : 0);
+  std::lock_guard<std::mutex> lock(g_hp_mutex);
+  int rc = 0;
+  if (!(g_hp.plan && g_hp.key == key)) {
+    g_hp.clear();
+    if (hipStreamCreate(&g_hp.s) != hipSuccess) return -1;
+    ApplyPlanInput in = {};
+    in.helm = helm;
+    in.npatches = npatches; in.npts = npts; in.nptso = nptso; in.nnz = nnz; in.nquad = nquad;
+    in.norders = norders; in.ixyzs = ixyzs; in.novers = novers; in.ixyzso = ixyzso;
+    in.row_ptr = row_ptr; in.col_ind = col_ind; in.iquad = iquad;
+    in.srcvals = srcvals; in.srcover = srcover; in.whtsover = whtsover;
+    in.wnear = wnear; in.wnear_is_device = false;
+    in.zk[0] = kpar6[0]; in.zk[1] = kpar6[1];
+    in.alpha[0] = kpar6[2]; in.alpha[1] = kpar6[3];
+    in.beta[0] = kpar6[4]; in.beta[1] = kpar6[5];
+    in.targs = targs;
+    in.ndtarg = ndtarg;
+    in.ntarg = nt_eff;
+    g_hp.plan = new ApplyPlan();
+    rc = apply_plan_build(g_hp.plan, in, g_hp.s);
+    g_hp.ns = (helm ? 2 : 1) * (size_t)npts;
+    g_hp.nt = (helm ? 2 : 1) * (size_t)nt_eff;
+    if (rc == 0 && (hipMalloc((void**)&g_hp.ds, (g_hp.ns ? g_hp.ns : 1) * 8) != hipSuccess ||
+                    hipMalloc((void**)&g_hp.dp, (g_hp.nt ? g_hp.nt : 1) * 8) != hipSuccess))
+      rc = -2;
+    if (rc != 0) {
+      g_hp.clear();
+      g_last_error = rc;
+      return rc;
+    }
+    g_hp.key = key;
+    g_hp.dev = dev;
+    g_hp_builds++;
+  }
+  hipStream_t s = g_hp.s;
+  ApplyPlan* P = g_hp.plan;
+  if (g_hp.ns && hipMemcpyAsync(g_hp.ds, sigma, g_hp.ns * 8, hipMemcpyHostToDevice, s) != hipSuccess) rc = -1;
+  if (rc == 0)
+    rc = helm ? apply_plan_run(P, 64, APPLY_ADDSUB, g_hp.ds, g_hp.dp, 1, s, nullptr)
+              : apply_plan_run_real(P, APPLY_ADDSUB, g_hp.ds, g_hp.dp, s);
+  if (rc == 0 && g_hp.nt && hipMemcpyAsync(pot, g_hp.dp, g_hp.nt * 8, hipMemcpyDeviceToHost, s) != hipSuccess)
+    rc = -1;
+  if (hipStreamSynchronize(s) != hipSuccess && rc == 0) rc = -1;
   g_last_error = rc;
   return rc;
 }
@@ -534,7 +618,8 @@ int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatc
                           const double* kpar6, int64_t nnz, const int64_t* row_ptr, const int64_t* col_ind,
                           const int64_t* iquad, double rfac0, double* wnear, int self_n, double self_tau_panel,
                           void* stream, double* stats, int* nref_host, int* class_host, const float* mlp_params,
-                          float* feat_out, int* pred_host) {
+                          float* feat_out, int* pred_host, const int* route_lmax, int* lev_host,
+                          int* unc_host) {
   NearQuadArgs A = {};
   A.helm = helm != 0;
   A.norder = norder;
@@ -556,15 +641,18 @@ int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatc
   A.mlp_params = mlp_params;
   A.feat_out = feat_out;
   A.pred_host = pred_host;
+  A.route_lmax = route_lmax;
+  A.lev_host = lev_host;
+  A.unc_host = unc_host;
   NearQuadStats st = {};
   const int rc = fmm_nearquad_device(A, &st, (hipStream_t)stream);
   if (stats) {
-    const double v[20] = {(double)st.nself,     (double)st.nadap,     (double)st.nfixed, (double)st.overflow,
-                          (double)st.nf,        (double)st.kq,        st.ms_prep,        st.ms_fixed,
-                          st.ms_adap,           st.ms_self,           st.ms_tables,      st.ms_select,
-                          st.ms_tree,           st.ms_shallow,        (double)st.nshallow0,
-                          (double)st.nshallow1, (double)st.nfallback, 0,                 0,
-                          0};
+    const double v[24] = {(double)st.nself,  (double)st.nadap,   (double)st.nfixed,  (double)st.overflow,
+                          (double)st.nf,     (double)st.kq,      st.ms_prep,         st.ms_fixed,
+                          st.ms_adap,        st.ms_self,         st.ms_tables,       st.ms_select,
+                          st.ms_tree,        st.ms_shallow,      (double)st.nroute,  0,
+                          (double)st.nfallback, (double)st.ncls[0], (double)st.ncls[1], (double)st.ncls[2],
+                          (double)st.ncls[3], (double)st.ncls[4], (double)st.ncls[5], 0};
     std::memcpy(stats, v, sizeof(v));
   }
   g_last_error = rc;
@@ -1022,8 +1110,104 @@ int fmm3dbie_far_order_dev(double eps, int64_t npatches, int norder, const int64
   return rc;
 }
 
-// Free the device work buffers the near-quadrature calls keep between calls.
-void fmm3dbie_release_workspace() { fmm_nearquad_release(); }
+// oversample_geom (surf_routs.f90:304-449) with host arrays, computed on the GPU.
+void oversample_geom_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs, const int64_t* iptype,
+                      const int64_t* npts, const double* srccoefs, const double* srcvals, const int64_t* nfars,
+                      const int64_t* ixyzso, const int64_t* nptso, double* srcover) {
+  const int64_t np = *npatches, N = *npts, NO = *nptso;
+  int rc = np > 0 ? uniform_order(np, norders, ixyzs, iptype) : 0;
+  hipStream_t s = nullptr;
+  if (rc >= 0 && hipStreamCreate(&s) != hipSuccess) rc = -1;
+  std::vector<void*> bufs;
+  auto up = [&](auto* h, size_t n) {
+    auto* d = upload(h, n, s);
+    bufs.push_back((void*)d);
+    if (!d) rc = -2;
+    return d;
+  };
+  if (rc >= 0 && np > 0) {
+    const int norder = rc;
+    const int64_t* d_ix = up(ixyzs, np + 1);
+    const double* d_sc = up(srccoefs, 9 * N);
+    const double* d_sv = up(srcvals, 12 * N);
+    const int64_t* d_nf = up(nfars, np);
+    const int64_t* d_ixo = up(ixyzso, np + 1);
+    double *d_so = nullptr, *d_wo = nullptr;
+    if (rc >= 0 && (hipMalloc(&d_so, sizeof(double) * 12 * (NO > 0 ? NO : 1)) != hipSuccess ||
+                    hipMalloc(&d_wo, sizeof(double) * (NO > 0 ? NO : 1)) != hipSuccess))
+      rc = -2;
+    bufs.push_back(d_so);
+    bufs.push_back(d_wo);
+    if (rc >= 0) rc = oversample_geom_device(np, norder, d_ix, d_sc, d_sv, d_nf, d_ixo, d_so, d_wo, s);
+    if (rc >= 0 && hipMemcpy(srcover, d_so, sizeof(double) * 12 * NO, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+  }
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (s) (void)hipStreamDestroy(s);
+  g_last_error = rc < 0 ? rc : 0;
+}
+
+// get_qwts (surf_routs.f90:1598-1651) with host arrays, computed on the GPU: |x_u x x_v|
+// times the RV weight of every discretization node.
+void get_qwts_(const int64_t* npatches, const int64_t* norders, const int64_t* ixyzs, const int64_t* iptype,
+               const int64_t* npts, const double* srcvals, double* qwts) {
+  const int64_t np = *npatches, N = *npts;
+  int rc = 0;
+  for (int64_t i = 0; i < np; i++) {  // patches of any order 0..20 (one RV rule each)
+    const int64_t p = (norders[i] + 1) * (norders[i] + 2) / 2;
+    if ((iptype && iptype[i] != 1) || norders[i] < 0 || norders[i] > 20 || ixyzs[i + 1] - ixyzs[i] != p) {
+      fprintf(stderr, "fmm3dbie_amd: get_qwts: patch %lld is not a triangle of order 0..20\n", (long long)i);
+      rc = -8;
+      break;
+    }
+  }
+  hipStream_t s = nullptr;
+  if (rc >= 0 && hipStreamCreate(&s) != hipSuccess) rc = -1;
+  std::vector<void*> bufs;
+  if (rc >= 0 && np > 0) {
+    int64_t *d_ix = upload(ixyzs, np + 1, s), *d_nf = upload(norders, np, s);
+    double* d_sv = upload(srcvals, 12 * N, s);
+    double *d_so = nullptr, *d_w = nullptr;
+    bufs = {d_ix, d_nf, d_sv};
+    if (!d_ix || !d_nf || !d_sv || hipMalloc(&d_so, sizeof(double) * 12 * (N > 0 ? N : 1)) != hipSuccess ||
+        hipMalloc(&d_w, sizeof(double) * (N > 0 ? N : 1)) != hipSuccess)
+      rc = -2;
+    bufs.push_back(d_so);
+    bufs.push_back(d_w);
+    // norder -1: every patch copies its own nodes and returns their weights (srccoefs
+    // is not read)
+    if (rc >= 0) rc = oversample_geom_device(np, -1, d_ix, d_sv, d_sv, d_nf, d_ix, d_so, d_w, s);
+    if (rc >= 0 && hipMemcpy(qwts, d_w, sizeof(double) * N, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+  }
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (s) (void)hipStreamDestroy(s);
+  g_last_error = rc < 0 ? rc : 0;
+}
+
+// oversample_geom + get_qwts of the oversampled nodes on device arrays.
+int fmm3dbie_oversample_geom_dev(int64_t npatches, int norder, const int64_t* ixyzs, const double* srccoefs,
+                                 const double* srcvals, const int64_t* nfars, const int64_t* ixyzso,
+                                 double* srcover, double* whtsover, void* stream) {
+  const int rc = oversample_geom_device(npatches, norder, ixyzs, srccoefs, srcvals, nfars, ixyzso, srcover,
+                                        whtsover, (hipStream_t)stream);
+  g_last_error = rc;
+  return rc;
+}
+
+// Free the device work buffers the near-quadrature calls keep between calls and the apply
+// plan the host lpcomp drop-ins keep.
+void fmm3dbie_release_workspace() {
+  fmm_nearquad_release();
+  std::lock_guard<std::mutex> lock(g_hp_mutex);
+  g_hp.clear();
+}
+
+// Apply plans built by the host lpcomp drop-ins since the library was loaded.
+int64_t fmm3dbie_host_plan_builds() {
+  std::lock_guard<std::mutex> lock(g_hp_mutex);
+  return g_hp_builds;
+}
 
 // Concurrency of the near-quadrature kernels (see the header).
 void fmm3dbie_set_side_stream(int enable) { fmm_nearquad_set_side_stream(enable); }

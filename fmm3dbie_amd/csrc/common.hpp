@@ -222,6 +222,44 @@ __device__ __forceinline__ void geom_from_row(const double* __restrict__ sc, con
   }
 }
 
+// Basis into an LDS staging row and geometry from the same values as they are generated
+// (no read-back of the row): g[m] = sum_l sc[9l+m] P_l(u,v), summed in generation order.
+template <int N>
+__device__ __forceinline__ void koorn_geom_fused(double u, double v, const double* __restrict__ sc, double* g,
+                                                 double* row) {
+  const double z = 2.0 * u + v - 1.0, y = 1.0 - v, x = 1.0 - 2.0 * v;
+  const double y2 = y * y;
+#pragma unroll
+  for (int m = 0; m < 9; m++) g[m] = 0.0;
+  double lkm1 = 0.0, lk = 1.0;
+#pragma unroll
+  for (int k = 0; k <= N; k++) {
+    if (k == 1) { lkm1 = 1.0; lk = z; }
+    if (k >= 2) {
+      const double ln = kTab.la[k - 1] * z * lk + kTab.lb[k - 1] * lkm1 * y2;
+      lkm1 = lk;
+      lk = ln;
+    }
+    double jm1 = 0.0, j0 = 1.0;
+#pragma unroll
+    for (int m = 0; m <= N - k; m++) {
+      const int n = m + k;
+      const int idx = n * (n + 1) / 2 + k;
+      const double val = lk * j0 * kTab.rs[m][k];
+      row[idx] = val;
+#pragma unroll
+      for (int c = 0; c < 9; c++) g[c] = fma(sc[9 * idx + c], val, g[c]);
+      if (m + 1 <= N - k) {
+        double jn;
+        if (m == 0) jn = kTab.ja[k][1] * x + kTab.jb[k][1];
+        else jn = (kTab.ja[k][m + 1] * x + kTab.jb[k][m + 1]) * j0 + kTab.jc[k][m + 1] * jm1;
+        jm1 = j0;
+        j0 = jn;
+      }
+    }
+  }
+}
+
 template <int N, bool SC_LDS = false>
 __device__ __forceinline__ void koorn_geom_row(double u, double v, const double* __restrict__ sc, double* g,
                                                double* row) {

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <map>
 #include <cstdio>
 #include <vector>
 
@@ -330,4 +331,136 @@ int far_order_device(const FarOrderArgs& A, hipStream_t s) {
       fprintf(stderr, "fmm3dbie_amd: get_far_order on the GPU supports patch orders 1..8 (got %d)\n", A.norder);
       return -4;
   }
+}
+
+// ---------------------------------------------------------------- oversampled geometry
+// oversample_geom (surf_routs.f90:304-449) and get_qwts (surf_routs.f90:1598-1651) on the
+// GPU, one workgroup per patch: the geometry at the RV nodes of the patch's oversampling
+// order nfar is the patch expansion times the basis at those nodes (the patch's own nodes
+// are copied when nfar equals the patch order), with unit normals; the node shared by the
+// two RV rules when both orders are multiples of 3 (get_clashing_indices,
+// patch_basis_routs.f90:518-572: node 1, or 76 for order 15 and 106 for order 18) takes
+// the patch's own values; the smooth weight of every node is |x_u x x_v| times the RV
+// weight of order nfar.
+namespace {
+
+constexpr int OV_MAXORD = 20;
+
+// Basis tables of one patch order: for every oversampling order nf <= OV_MAXORD, the
+// npols(norder) basis values at each of the npols(nf) RV nodes of order nf (off[nf] is
+// the first row) and the RV weights of order nf (woff[nf]).
+struct OversampleTables {
+  int norder = -1;
+  double* pmat = nullptr;
+  double* wts = nullptr;
+  int64_t off[OV_MAXORD + 2] = {}, woff[OV_MAXORD + 2] = {};
+};
+
+__device__ __forceinline__ int clash_node(int n) { return n == 15 ? 75 : (n == 18 ? 105 : 0); }
+
+__global__ void k_oversample_geom(int64_t npatches, int norder, int np, const int64_t* __restrict__ ixyzs,
+                                  const double* __restrict__ srccoefs, const double* __restrict__ srcvals,
+                                  const int64_t* __restrict__ nfars, const int64_t* __restrict__ ixyzso,
+                                  const double* __restrict__ pmat, const int64_t* __restrict__ off,
+                                  const double* __restrict__ rvw, const int64_t* __restrict__ woff,
+                                  double* __restrict__ srcover, double* __restrict__ wover) {
+  const int64_t ip = blockIdx.x;
+  if (ip >= npatches) return;
+  // norder < 0: every patch keeps its own nodes (the smooth weights of a surface whose
+  // patch orders are nfars, get_qwts)
+  const int nf = (int)nfars[ip];
+  const bool own = norder < 0 || nf == norder;
+  const int64_t i0 = ixyzs[ip] - 1, o0 = ixyzso[ip] - 1;
+  const int npo = (int)(ixyzso[ip + 1] - ixyzso[ip]);
+  const double* sc = srccoefs + 9 * i0;
+  const bool clash = !own && norder % 3 == 0 && nf % 3 == 0;
+  const int jc = clash_node(nf), ic = clash_node(norder);
+  for (int j = threadIdx.x; j < npo; j += blockDim.x) {
+    double g[12];
+    if (own || (clash && j == jc)) {
+      const double* sv = srcvals + 12 * (i0 + (own ? j : ic));
+      for (int m = 0; m < 12; m++) g[m] = sv[m];
+    } else {
+      const double* P = pmat + (off[nf] + j) * (int64_t)np;
+      for (int m = 0; m < 9; m++) g[m] = 0.0;
+      for (int l = 0; l < np; l++)
+        for (int m = 0; m < 9; m++) g[m] = fma(sc[9 * l + m], P[l], g[m]);
+      const double nx = g[4] * g[8] - g[5] * g[7], ny = g[5] * g[6] - g[3] * g[8], nz = g[3] * g[7] - g[4] * g[6];
+      const double rr = sqrt(nx * nx + ny * ny + nz * nz);
+      g[9] = nx / rr;
+      g[10] = ny / rr;
+      g[11] = nz / rr;
+    }
+    double* o = srcover + 12 * (o0 + j);
+    for (int m = 0; m < 12; m++) o[m] = g[m];
+    const double nx = g[4] * g[8] - g[5] * g[7], ny = g[5] * g[6] - g[3] * g[8], nz = g[3] * g[7] - g[4] * g[6];
+    wover[o0 + j] = sqrt(nx * nx + ny * ny + nz * nz) * rvw[woff[nf] + j];
+  }
+}
+
+std::map<std::pair<int, int>, OversampleTables> g_ov;  // (device, patch order) -> tables
+
+int oversample_tables(int norder, const OversampleTables** out, hipStream_t s) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  auto it = g_ov.find({dev, norder});
+  if (it == g_ov.end()) {
+    OversampleTables T;
+    T.norder = norder;
+    const int np = (norder + 1) * (norder + 2) / 2;
+    std::vector<double> pm, w, uv, rw, pols(np);
+    for (int nf = 0; nf <= OV_MAXORD; nf++) {
+      const int npf = (nf + 1) * (nf + 2) / 2;
+      T.off[nf] = (int64_t)pm.size() / np;
+      T.woff[nf] = (int64_t)w.size();
+      uv.assign(2 * npf, 0.0);
+      rw.assign(npf, 0.0);
+      fmmhost::rv_nodes(nf, uv.data(), rw.data());
+      for (int j = 0; j < npf; j++) {
+        fmmhost::koorn_pols(uv[2 * j], uv[2 * j + 1], norder, pols.data());
+        pm.insert(pm.end(), pols.begin(), pols.end());
+        w.push_back(rw[j]);
+      }
+    }
+    T.off[OV_MAXORD + 1] = (int64_t)pm.size() / np;
+    T.woff[OV_MAXORD + 1] = (int64_t)w.size();
+    if (hipMalloc(&T.pmat, sizeof(double) * pm.size()) != hipSuccess) return -2;
+    if (hipMalloc(&T.wts, sizeof(double) * w.size()) != hipSuccess) return -2;
+    if (hipMemcpyAsync(T.pmat, pm.data(), sizeof(double) * pm.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(T.wts, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return -1;
+    it = g_ov.emplace(std::make_pair(dev, norder), T).first;
+  }
+  *out = &it->second;
+  return 0;
+}
+
+}  // namespace
+
+int oversample_geom_device(int64_t npatches, int norder, const int64_t* ixyzs, const double* srccoefs,
+                           const double* srcvals, const int64_t* nfars, const int64_t* ixyzso, double* srcover,
+                           double* wover, hipStream_t s) {
+  if (npatches == 0) return 0;
+  if (norder > OV_MAXORD) return -4;
+  const OversampleTables* T = nullptr;
+  const int rc = oversample_tables(norder < 0 ? 0 : norder, &T, s);
+  if (rc) return rc;
+  int64_t* doff = nullptr;
+  if (hipMalloc(&doff, sizeof(int64_t) * 2 * (OV_MAXORD + 2)) != hipSuccess) return -2;
+  int64_t hoff[2 * (OV_MAXORD + 2)];
+  for (int k = 0; k < OV_MAXORD + 2; k++) {
+    hoff[k] = T->off[k];
+    hoff[OV_MAXORD + 2 + k] = T->woff[k];
+  }
+  int err = 0;
+  if (hipMemcpyAsync(doff, hoff, sizeof(hoff), hipMemcpyHostToDevice, s) != hipSuccess) err = -1;
+  if (!err) {
+    const int np = norder < 0 ? 1 : (norder + 1) * (norder + 2) / 2;
+    k_oversample_geom<<<(unsigned)npatches, 64, 0, s>>>(npatches, norder, np, ixyzs, srccoefs, srcvals, nfars, ixyzso,
+                                                        T->pmat, doff, T->wts, doff + OV_MAXORD + 2, srcover, wover);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) err = -1;
+  }
+  (void)hipFree(doff);
+  return err;
 }

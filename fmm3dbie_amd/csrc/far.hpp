@@ -24,3 +24,11 @@ struct FarOrderArgs {
 };
 
 int far_order_device(const FarOrderArgs& a, hipStream_t s);
+
+// oversample_geom + the smooth weights of the oversampled nodes (get_qwts on srcover) on
+// device arrays; nfars (npatches) are the oversampling orders (<= 20), ixyzso their point
+// offsets (1-based); srcover (12, nptso), wover (nptso) out.  norder < 0: every patch
+// copies its own nodes (nfars = the patch orders; wover = get_qwts).
+int oversample_geom_device(int64_t npatches, int norder, const int64_t* ixyzs, const double* srccoefs,
+                           const double* srcvals, const int64_t* nfars, const int64_t* ixyzso, double* srcover,
+                           double* wover, hipStream_t s);

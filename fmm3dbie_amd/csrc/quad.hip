@@ -6,23 +6,28 @@
 //                       tolerance epsp = eps_adap*rsc^(1/4) (ggq-quads.f:329-337).
 //   2. k_classify     : per CSR entry -> self / adaptive (|x-c| <= rfac0*rad) / fixed
 //                       (ggq-quads.f:341-365, get_near_far_split_pt surf_routs.f90:1653).
-//   3. k_fixed        : one workgroup per source patch; the patch's oversampled rule
-//                       geometry is built once in LDS and every far-near target of the
-//                       patch is integrated against it (ctriaints_wnodes,
-//                       triaintrouts.f:67-271 + umat^T, ggq-quads.f:412).
-//   4. k_adaptive     : persistent wavefronts pull (target, patch) pairs from a queue
+//   3. k_fixed_mfma   : one workgroup per source patch; the patch's oversampled rule
+//      (k_fixed)        geometry is built once in LDS and every far-near target of the
+//                       patch is integrated against it, 16 targets per wavefront with
+//                       the moments on FP64 MFMA (ctriaints_wnodes, triaintrouts.f:67-271
+//                       + umat^T, ggq-quads.f:412).
+//   4. k_tree_cache   : node geometry of the refinement tree's depths 0..3 per patch.
+//   5. k_adaptive     : persistent wavefronts pull (target, patch) pairs from a queue
 //                       and run the 1->4 refinement of triaadap_main
 //                       (triaintrouts.f:1269-1464) depth-first with a per-wave stack in
-//                       HBM; 64 lanes evaluate the XG nodes of one child at a time.
-//   5. k_self         : one wavefront per on-surface target; a polar/sinh product rule
+//                       HBM; the four children of a step are evaluated together, 64
+//                       nodes at a time.  With learned order selection each pair runs
+//                       the XG rule of its predicted class.
+//   6. k_self         : one wavefront per on-surface target; a polar/sinh product rule
 //                       on the three triangles meeting at the target replaces the GGQ
 //                       rule of zget_ggq_self_quad_pt (ggq-quads.f:933-1122).
-// Moments are formed by staging (f_q, P_j(q)) rows in LDS and letting lane c sum
-// column c, then weights = umat^T * moments.
+// Moments are formed by staging kernel values f_q and basis rows P_j(q) in LDS and
+// letting lane j sum column j, then weights = umat^T * moments.
 #include <hip/hip_runtime.h>
 
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -300,6 +305,105 @@ __global__ void __launch_bounds__(128) k_fixed(int nf, const double* __restrict_
   }
 }
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Fixed rule on the matrix cores: the weights of a patch's far-near targets are the
+// product W = F B of the kernel matrix F (targets x nf nodes, F_tq = K(x_t, y_q) |J_q|)
+// and the patch-independent table B (nf x NP, B_qj = w_q sum_l P_l(q) umat(l, j)), i.e.
+// ctriaints_wnodes (triaintrouts.f:67-271) followed by umat^T (ggq-quads.f:412).  A
+// workgroup takes one source patch: its rule geometry is built once into LDS, then each
+// wavefront takes 16 targets at a time.  In a step of 4 nodes, lane l evaluates F for
+// target l % 16 at node l / 16 (64 independent kernel evaluations) and the FP64 MFMA
+// 16x16x4 multiplies that 16x4 block of F (real and imaginary parts) by the 4x16 block
+// of B (basis functions 16 ct .. 16 ct + 15); accumulator element r of lane l holds
+// target l / 16 + 4 r, basis function l % 16 + 16 ct.
+template <int N, bool HELM>
+__global__ void __launch_bounds__(256) k_fixed_mfma(int nf, const double* __restrict__ Pq,
+                                                    const double* __restrict__ Bq, const int64_t* __restrict__ ixyzs,
+                                                    const double* __restrict__ srccoefs, int64_t ndtarg,
+                                                    const double* __restrict__ targs,
+                                                    const int64_t* __restrict__ iquad, const int* __restrict__ off,
+                                                    const int2* __restrict__ fixed_list, KPar kp,
+                                                    double* __restrict__ wnear) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  constexpr int CT = (NP + 15) / 16;
+  constexpr int NPART = HELM ? 2 : 1;
+  double* lds = fmm_dyn_lds;
+  const int ip = blockIdx.x;
+  const int a = off[ip], b = off[ip + 1];
+  if (a == b) return;
+  double* sc = lds;             // 9*NP
+  double* node = lds + 9 * NP;  // nf * 8: xyz, n, |J|, pad
+  const double* gsc = srccoefs + 9 * (ixyzs[ip] - 1);
+  for (int i = threadIdx.x; i < 9 * NP; i += blockDim.x) sc[i] = gsc[i];
+  __syncthreads();
+  for (int q = threadIdx.x; q < nf; q += blockDim.x) {
+    double P[NP], g[12];
+#pragma unroll
+    for (int l = 0; l < NP; l++) P[l] = Pq[q * NP + l];
+    const double jac = geom_eval<NP>(sc, P, g);
+    double* o = node + 8 * q;
+    o[0] = g[0]; o[1] = g[1]; o[2] = g[2];
+    o[3] = g[9]; o[4] = g[10]; o[5] = g[11];
+    o[6] = jac;
+    o[7] = 0.0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int ia = lane & 15, ka = lane >> 4;
+  const int ntile = (b - a + 15) / 16;
+  for (int tile = wv; tile < ntile; tile += nw) {
+    const int k0 = a + 16 * tile;
+    const bool row_ok = k0 + ia < b;
+    const int2 jt = fixed_list[row_ok ? k0 + ia : k0];
+    const double* tp = targs + ndtarg * (int64_t)jt.y;
+    const double tx = tp[0], ty = tp[1], tz = tp[2];
+    dbl4 acc[NPART][CT];
+#pragma unroll
+    for (int pa = 0; pa < NPART; pa++)
+#pragma unroll
+      for (int ct = 0; ct < CT; ct++) acc[pa][ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int q0 = 0; q0 < nf; q0 += 4) {
+      const int q = q0 + ka;  // nf is a multiple of 4 (4^nlev XG triangles)
+      const double* o = node + 8 * q;
+      double vr, vi;
+      if (HELM) {
+        helm_comb(tx - o[0], ty - o[1], tz - o[2], o[3], o[4], o[5], kp, vr, vi);
+      } else {
+        vr = lap_comb(tx - o[0], ty - o[1], tz - o[2], o[3], o[4], o[5], kp);
+        vi = 0.0;
+      }
+      const double fr = row_ok ? vr * o[6] : 0.0, fi = row_ok ? vi * o[6] : 0.0;
+#pragma unroll
+      for (int ct = 0; ct < CT; ct++) {
+        const int j = ia + 16 * ct;
+        const double bb = j < NP ? Bq[q * NP + j] : 0.0;
+        acc[0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, bb, acc[0][ct], 0, 0, 0);
+        if (HELM) acc[NPART - 1][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fi, bb, acc[NPART - 1][ct], 0, 0, 0);
+      }
+    }
+    // accumulator element r: target k0 + ka + 4 r, basis function ia + 16 ct
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int k = k0 + ka + 4 * r;
+      if (k >= b) continue;
+      const int64_t q0 = iquad[fixed_list[k].x] - 1;
+#pragma unroll
+      for (int ct = 0; ct < CT; ct++) {
+        const int j = ia + 16 * ct;
+        if (j >= NP) continue;
+        if (HELM) {
+          wnear[2 * (q0 + j)] = acc[0][ct][r];
+          wnear[2 * (q0 + j) + 1] = acc[NPART - 1][ct][r];
+        } else {
+          wnear[q0 + j] = acc[0][ct][r];
+        }
+      }
+    }
+  }
+}
+
 // LDS staging row of one node: f_re, f_im, P_0..P_{NP-1}, padded to an even number of
 // doubles so that every row starts on a 16-byte boundary and (f_re, f_im) is one 128-bit read.
 __host__ __device__ constexpr int row_w(int np) { return (np + 3) & ~1; }
@@ -374,35 +478,6 @@ __device__ __forceinline__ void store_weights(double* buf, Mom mom, const double
 }
 
 // ------------------------------------------------------------------ 4. adaptive
-// Moments over one tree triangle (x0,y0) + diag(dx,dy)*(u,v) of the XG(kq) rule.
-template <int N, bool HELM>
-__device__ __forceinline__ Mom tri_moments(double* buf, const double* sc, double x0, double y0, double dx,
-                                           double dy, double da, int kq, const double* __restrict__ xg,
-                                           const double* tg, const KPar& kp) {
-  constexpr int NP = (N + 1) * (N + 2) / 2;
-  constexpr int W = row_w(NP);
-  const int lane = threadIdx.x;
-  Mom total{0.0, 0.0};
-  for (int base = 0; base < kq; base += WAVE) {
-    const int q = base + lane;
-    if (q < kq) {
-      double* row = buf + lane * W;
-      const double u = x0 + dx * xg[3 * q], v = y0 + dy * xg[3 * q + 1];
-      double g[12];
-      koorn_geom_row<N, true>(u, v, sc, g, row + 2);
-      const double jac = geom_finish(g);
-      const double w = jac * da * xg[3 * q + 2];
-      double vr, vi;
-      kern_eval<HELM>(tg, g, kp, vr, vi);
-      row[0] = vr * w;
-      row[1] = vi * w;
-    }
-    const int nrows = kq - base < WAVE ? kq - base : WAVE;
-    total += column_sum<NP, HELM>(buf, nrows);
-  }
-  return total;
-}
-
 // Depths 0..3 of the refinement tree (1 + 4 + 16 + 64 triangles: root 0, children of
 // the root 1..4, children of t in 1..4 at 5 + 4(t-1) + c, children of t in 5..20 at
 // 21 + 4(t-5) + c) are the same for every target of a patch, as in the reference's
@@ -413,24 +488,34 @@ constexpr int NCACHE_TRI = 85;
 constexpr int CACHE_STRIDE = 8;
 constexpr int NCACHE_PARENT = 21;  // triangles whose children are cached (depths 0..2)
 
+// Rows of the cached-tree basis table are padded to an even length, so every row starts on
+// a 16-byte boundary and is read with 16-byte loads.
+__host__ __device__ constexpr int ptab_w(int np) { return (np + 1) & ~1; }
+
 __host__ __device__ __forceinline__ int cached_child(int t, int c) {
   return t < 0 ? -1 : (t == 0 ? 1 + c : (t <= 4 ? 5 + 4 * (t - 1) + c : (t <= 20 ? 21 + 4 * (t - 5) + c : -1)));
 }
 
+// The cache of a rule may hold fewer depths (ntri = 1 + 4 + ... triangles of the first
+// depths, in tree order): a rule class whose pairs stop at depth 1 or 2 needs no more.
+__host__ __device__ constexpr int cache_tris(int npar) { return 4 * npar + 1; }
+__host__ __device__ constexpr int cache_parents(int depth) { return depth <= 1 ? 1 : (depth == 2 ? 5 : NCACHE_PARENT); }
+
 template <int N>
-__global__ void k_tree_cache(int64_t npatches, int kq, const double* __restrict__ tuv, const double* __restrict__ twq,
-                             const double* __restrict__ ptab, const int64_t* __restrict__ ixyzs,
-                             const double* __restrict__ srccoefs, double* __restrict__ cache) {
+__global__ void k_tree_cache(int64_t npatches, int kq, int ntri, const double* __restrict__ tuv,
+                             const double* __restrict__ twq, const double* __restrict__ ptab,
+                             const int64_t* __restrict__ ixyzs, const double* __restrict__ srccoefs,
+                             double* __restrict__ cache) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
   const int64_t ip = blockIdx.x;
-  const int nn = NCACHE_TRI * kq;
+  const int nn = ntri * kq;
   const double* sc = srccoefs + 9 * (ixyzs[ip] - 1);
   double* out = cache + (size_t)ip * CACHE_STRIDE * nn;
   for (int q = threadIdx.x; q < nn; q += blockDim.x) {
     double g[12];
     for (int m = 0; m < 9; m++) g[m] = 0.0;
     for (int l = 0; l < NP; l++) {
-      const double pl = ptab[(size_t)q * NP + l];
+      const double pl = ptab[(size_t)q * ptab_w(NP) + l];
       for (int m = 0; m < 9; m++) g[m] = fma(sc[9 * l + m], pl, g[m]);
     }
     const double jac = geom_finish(g);
@@ -446,47 +531,147 @@ __global__ void k_tree_cache(int64_t npatches, int kq, const double* __restrict_
   }
 }
 
-// moments over cached tree triangle `tri`: lane q evaluates the kernel at node q and
-// stages (f_q, P_j(q)) in LDS, the basis values copied from the shared table; lane c
-// then sums column c
-template <int N, bool HELM>
-__device__ __forceinline__ Mom tri_moments_cached(double* buf, const double* __restrict__ cpatch, int nn,
-                                                  const double* __restrict__ ptab, int tri, int kq,
-                                                  const double* tg, const KPar& kp) {
-  constexpr int NP = (N + 1) * (N + 2) / 2;
+// Rules of the adaptive integrator, one per class: the XG rule (kq nodes), the node
+// geometry of the cached tree for that rule (per patch) and its basis table, and the
+// deepest level a pair of the class may reach.  The work list is sorted by class;
+// seg[c] .. seg[c + 1] - 1 are the pairs of class c.  A plain generation has one class:
+// the reference's rule and level cap (NLMAX).
+constexpr int MAX_RULES = 6;
+struct AdapRules {
+  int nrules;
+  int kq[MAX_RULES];
+  int lmax[MAX_RULES];
+  int npar[MAX_RULES];  // cached triangles whose children are cached too (1, 5 or 21)
+  int seg[MAX_RULES + 1];
+  const double* xg[MAX_RULES];
+  const double* cache[MAX_RULES];
+  const double* ptab[MAX_RULES];
+};
+
+// Column sums of staged rows r0 <= r < r1 (no barriers: the caller brackets the chunk):
+// row q holds the basis values P_0..P_{NP-1} of node q at pb + q * PW and its kernel value
+// f_q at fb[q].
+template <int NP, bool HELM>
+__device__ __forceinline__ Mom column_sum_rows(const double* pb, const double2* fb, int r0, int r1) {
+  constexpr int PW = ptab_w(NP);
+  constexpr int G = WAVE / NP;
   const int lane = threadIdx.x;
-  Mom total{0.0, 0.0};
-  for (int base = 0; base < kq; base += WAVE) {
-    const int q = base + lane;
-    if (q < kq) {
-      const int node = tri * kq + q;
-      double* row = buf + lane * row_w(NP);
-      for (int l = 0; l < NP; l++) row[2 + l] = ptab[(size_t)node * NP + l];
-      double g[12];
-      const double* cn = cpatch + (size_t)CACHE_STRIDE * node;
-      g[0] = cn[0];
-      g[1] = cn[1];
-      g[2] = cn[2];
-      g[9] = cn[3];
-      g[10] = cn[4];
-      g[11] = cn[5];
-      const double w = cn[6];
-      double vr, vi;
-      kern_eval<HELM>(tg, g, kp, vr, vi);
-      row[0] = vr * w;
-      row[1] = vi * w;
+  const int grp = lane / NP, j = lane - grp * NP;
+  double sr = 0.0, si = 0.0;
+  if (grp < G) {
+#pragma unroll 4
+    for (int q = r0 + grp; q < r1; q += G) {
+      const double p = pb[q * PW + j];
+      const double2 f = fb[q];
+      sr = fma(f.x, p, sr);
+      if (HELM) si = fma(f.y, p, si);
     }
-    const int nrows = kq - base < WAVE ? kq - base : WAVE;
-    total += column_sum<NP, HELM>(buf, nrows);
   }
-  return total;
+  const double pr = sr, pi = si;
+#pragma unroll
+  for (int k = 1; k < G; k++) {
+    sr += __shfl(pr, (lane + k * NP) & 63, 64);
+    if (HELM) si += __shfl(pi, (lane + k * NP) & 63, 64);
+  }
+  return lane < NP ? Mom{sr, HELM ? si : 0.0} : Mom{0.0, 0.0};
 }
 
+// Moments of the triangles of one refinement step (root: the patch triangle, m[0];
+// otherwise the four children of the triangle (x0, y0, dx, dy) at tree id tid, m[0..3]):
+// their ntri * kq nodes are spread over the lanes 64 at a time, so a small rule still
+// fills the wavefront and the kernel evaluations of all children are in flight together.
+// Children of a cached triangle have consecutive tree ids, so a chunk's basis rows are
+// one contiguous block of the basis table: it is copied to LDS by the memory unit
+// (64 x 16 bytes per load-to-LDS instruction, no registers) while the lanes read the
+// nodes' geometry from the tree cache and evaluate the kernel.  Below the cached depths
+// each lane writes its node's basis row and sums the geometry from it and the patch
+// coefficients sc in LDS.
 template <int N, bool HELM>
-__global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restrict__ list,
-                                                 const int2* __restrict__ fb_list, const int* __restrict__ fb_n,
-                                                 int* __restrict__ qctr, int kq, const double* __restrict__ xg,
-                                                 const int64_t* __restrict__ col_ind,
+__device__ __forceinline__ void step_moments(double* pb, double2* fb, const double* sc, bool root, int npar,
+                                             int tid, double x0, double y0, double dx, double dy, int kq,
+                                             const double* __restrict__ xg, const double* __restrict__ cpatch,
+                                             const double* __restrict__ ptab, const double* tg, const KPar& kp,
+                                             Mom (&m)[4]) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  constexpr int PW = ptab_w(NP);
+  const int lane = threadIdx.x;
+  const int ntri = root ? 1 : 4;
+  const int total = ntri * kq;
+  // cached: the root itself, or children of a cached triangle of depth <= 2
+  const bool cached = root ? true : (tid >= 0 && tid < npar);
+  const int t0 = root ? 0 : cached_child(tid, 0);  // first cached tree id of the step
+  const double hx = 0.5 * dx, hy = 0.5 * dy, da = hx * hx;
+#pragma unroll
+  for (int c = 0; c < 4; c++) m[c] = Mom{0.0, 0.0};
+  for (int base = 0; base < total; base += WAVE) {
+    const int n = base + lane;
+    if (cached) {
+      // rows t0 kq + base .. + 63 of the table (padded by 64 rows past the last triangle)
+      const double* blk = ptab + ((size_t)t0 * kq + base) * PW;
+#pragma unroll
+      for (int k = 0; k < PW / 2; k++)
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void*)(blk + 2 * (k * WAVE + lane)),
+            (__attribute__((address_space(3))) void*)(pb + 2 * k * WAVE), 16, 0, 0);
+    }
+    if (n < total) {
+      const int c = ntri == 1 ? 0 : (n >= kq) + (n >= 2 * kq) + (n >= 3 * kq);
+      const int q = n - c * kq;
+      double g[12], w;
+      if (cached) {
+        const double* cn = cpatch + (size_t)CACHE_STRIDE * ((size_t)t0 * kq + n);
+        g[0] = cn[0];
+        g[1] = cn[1];
+        g[2] = cn[2];
+        g[9] = cn[3];
+        g[10] = cn[4];
+        g[11] = cn[5];
+        w = cn[6];
+      } else {
+        // child c: corner (x0, y0) + ((c & 1) hx, (c >> 1) hy), legs (hx, hy), the fourth reversed
+        const double cx = (c & 1) ? x0 + hx : x0, cy = (c & 2) ? y0 + hy : y0;
+        const double cdx = c == 3 ? -hx : hx, cdy = c == 3 ? -hy : hy;
+        const double u = cx + cdx * xg[3 * q], v = cy + cdy * xg[3 * q + 1];
+        koorn_geom_row<N, true>(u, v, sc, g, pb + lane * PW);
+        w = geom_finish(g) * da * xg[3 * q + 2];
+      }
+      double vr, vi;
+      kern_eval<HELM>(tg, g, kp, vr, vi);
+      fb[lane] = make_double2(vr * w, vi * w);
+    }
+    const int nrows = total - base < WAVE ? total - base : WAVE;
+    // the loads to LDS are counted by vmcnt: wait for them before the barrier
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __syncthreads();
+    // the rows of this chunk belong to triangles c0..c1 (wave-uniform)
+    const int c0 = ntri == 1 ? 0 : base / kq, c1 = ntri == 1 ? 0 : (base + nrows - 1) / kq;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if (c < c0 || c > c1) continue;
+      const int r0 = c * kq - base > 0 ? c * kq - base : 0;
+      const int r1 = (c + 1) * kq - base < nrows ? (c + 1) * kq - base : nrows;
+      m[c] += column_sum_rows<NP, HELM>(pb, fb, r0, r1);
+    }
+    __syncthreads();
+  }
+}
+
+// Adaptive integration (triaadap_main, triaintrouts.f:1269-1464): persistent wavefronts
+// pull (target, patch) pairs from a queue; each pair's refinement tree is walked depth
+// first from a per-wavefront stack (the leaf set is that of the reference's level-by-level
+// walk: every decision depends only on its own triangle).  A step evaluates the four
+// children of the popped triangle together and refines iff max_j |sum_c M_c,j - M_j| >
+// epsp (triaintrouts.f:1435-1448); the children are pushed only when refined.  A pair of
+// a routed class whose tree needs leaves below the class's level cap (lmax < NLMAX) is
+// appended to fb_out
+// (the plain integrator takes it afterwards) and gets no weights here.  The list is read
+// from nlist or, when nlist_dev is set, from the device count (a second launch over the
+// pairs a first one sent back).
+template <int N, bool HELM>
+__global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restrict__ nlist_dev,
+                                                 const int2* __restrict__ list, AdapRules R,
+                                                 int2* __restrict__ fb_out, int* __restrict__ fb_cnt,
+                                                 int* __restrict__ qctr, const int64_t* __restrict__ col_ind,
                                                  const int64_t* __restrict__ ixyzs,
                                                  const double* __restrict__ srccoefs, int64_t ndtarg,
                                                  const double* __restrict__ targs,
@@ -494,32 +679,32 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
                                                  const double* __restrict__ epsp_patch,
                                                  const double* __restrict__ umat, KPar kp,
                                                  double* __restrict__ stack_all, int* __restrict__ nref_out,
-                                                 int* __restrict__ overflow, double* __restrict__ wnear,
-                                                 const double* __restrict__ cache, const double* __restrict__ ptab) {
+                                                 int* __restrict__ lev_out, int* __restrict__ unc_out,
+                                                 int* __restrict__ overflow, double* __restrict__ wnear) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
-  constexpr int ES = 6 + 2 * NP;  // stack entry: x0,y0,dx,dy,level,pad, moments
-  __shared__ __align__(16) double buf[WAVE * row_w(NP)];
+  constexpr int ES = 6 + 2 * NP;  // stack entry: x0,y0,dx,dy,level,tree id, moments
+  __shared__ __align__(16) double pb[WAVE * ptab_w(NP)];
+  __shared__ __align__(16) double2 fb[WAVE];
   __shared__ double lsc[9 * NP];
   __shared__ int sh_idx;
   const int lane = threadIdx.x;
   double* stack = stack_all + (size_t)blockIdx.x * STACK_CAP * ES;
-  // Work queue: the list (sorted by patch) in 8 contiguous segments, one per XCD
-  // (workgroups are dispatched to the XCDs round-robin, so workgroup b starts on segment
-  // b % 8 and the patch geometry it reads stays in its XCD's L2), then the other
-  // segments once its own is drained, then the pairs the cached-tree kernels sent back.
-  const int nfb = fb_n ? *fb_n : 0;
+  const int nl = nlist_dev ? *nlist_dev : nlist;
+  // Work queue: the list (sorted by class, then patch) in 8 contiguous segments, one per
+  // XCD (workgroups are dispatched to the XCDs round-robin, so workgroup b starts on
+  // segment b % 8 and the patch geometry it reads stays in its XCD's L2), then the other
+  // segments once its own is drained.
   const int home = blockIdx.x & 7;
   int tried = 0;  // segments found empty (lane 0)
   for (;;) {
     if (lane == 0) {
       int got = -1;
-      while (tried < 9) {
-        const int sg = tried < 8 ? (home + tried) & 7 : 8;
-        const int a = sg < 8 ? (int)((int64_t)nlist * sg / 8) : 0;
-        const int b = sg < 8 ? (int)((int64_t)nlist * (sg + 1) / 8) : nfb;
+      while (tried < 8) {
+        const int sg = (home + tried) & 7;
+        const int a = (int)((int64_t)nl * sg / 8), b = (int)((int64_t)nl * (sg + 1) / 8);
         const int k = atomicAdd(qctr + sg, 1);
         if (a + k < b) {
-          got = sg < 8 ? a + k : nlist + k;
+          got = a + k;
           break;
         }
         tried++;
@@ -527,86 +712,98 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int2* __restri
       sh_idx = got;
     }
     __syncthreads();
-    // wave-uniform from here: the pair, its patch and the patch coefficients are read
-    // through scalar loads into SGPRs
     const int idx = __builtin_amdgcn_readfirstlane(sh_idx);
     __syncthreads();
     if (idx < 0) break;
-    const int2 jt = idx < nlist ? list[idx] : fb_list[idx - nlist];
+    int cls = 0;
+    while (cls + 1 < R.nrules && idx >= R.seg[cls + 1]) cls++;
+    cls = __builtin_amdgcn_readfirstlane(cls);
+    const int kq = R.kq[cls], lmax = R.lmax[cls];
+    const double* xg = R.xg[cls];
+    const double* ptab = R.ptab[cls];
+    const int2 jt = list[idx];
     const int64_t ip = col_ind[jt.x] - 1;
-    // patch coefficients staged in LDS for the geometry of the triangles below the cached
-    // tree (read after the barriers of the root triangle's moments)
     {
       const double* gsc = srccoefs + 9 * (ixyzs[ip] - 1);
       for (int i = lane; i < 9 * NP; i += WAVE) lsc[i] = gsc[i];
     }
-    const double* sc = lsc;
     const double* tp = targs + ndtarg * (int64_t)jt.y;
     const double tg[3] = {tp[0], tp[1], tp[2]};
     const double eps = epsp_patch[ip];
-    const int nn = NCACHE_TRI * kq;
-    const double* cpatch = cache + (size_t)ip * CACHE_STRIDE * nn;
-    // root triangle (cached tree id 0)
-    const Mom mroot = tri_moments_cached<N, HELM>(buf, cpatch, nn, ptab, 0, kq, tg, kp);
-    int sp = 0;
+    const int npar = R.npar[cls];
+    const double* cpatch = R.cache[cls] + (size_t)ip * CACHE_STRIDE * cache_tris(npar) * kq;
+    Mom mc[4];
+    step_moments<N, HELM>(pb, fb, lsc, true, npar, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
     if (lane < 6) stack[lane] = lane < 2 ? 0.0 : (lane < 4 ? 1.0 : 0.0);  // level 0, tree id 0
     if (lane < NP) {
-      stack[6 + lane] = mroot.re;
-      stack[6 + NP + lane] = mroot.im;
+      stack[6 + lane] = mc[0].re;
+      stack[6 + NP + lane] = mc[0].im;
     }
-    sp = 1;
+    int sp = 1, nref = 0, deep = 0, nunc = 0;
+    bool aborted = false;
     Mom result{0.0, 0.0};
-    int nref = 0;
     while (sp > 0) {
       sp--;
       const double* e = stack + (size_t)sp * ES;
-      // the entry's triangle is wave-uniform: kept in scalar registers
-      const double x0 = wave_uniform(e[0]), y0 = wave_uniform(e[1]), dx = wave_uniform(e[2]), dy = wave_uniform(e[3]);
+      const double x0 = wave_uniform(e[0]), y0 = wave_uniform(e[1]), dx = wave_uniform(e[2]),
+                   dy = wave_uniform(e[3]);
       const int level = __builtin_amdgcn_readfirstlane((int)e[4]);
       const int tid = __builtin_amdgcn_readfirstlane((int)e[5]);  // cached tree id, -1 below depth 2
       const Mom mpar = lane < NP ? Mom{e[6 + lane], e[6 + NP + lane]} : Mom{0.0, 0.0};
-      const double hx = 0.5 * dx, hy = 0.5 * dy, da = hx * hx;  // child area ratio 4^-(level+1)
-
-      Mom ssum{0.0, 0.0};
-      // the four children go to stack slots sp..sp+3 (committed below only if refined)
-      const bool room = sp + 4 <= STACK_CAP;
-#pragma unroll 1
-      for (int c = 0; c < 4; c++) {
-        const int ctid = cached_child(tid, c);
-        // child c: corner (x0, y0) + ((c & 1) hx, (c >> 1) hy), legs (hx, hy), the fourth reversed
-        const double cxc = (c & 1) ? x0 + hx : x0, cyc = (c & 2) ? y0 + hy : y0;
-        const double cdxc = c == 3 ? -hx : hx, cdyc = c == 3 ? -hy : hy;
-        const Mom mcc = ctid >= 0
-                               ? tri_moments_cached<N, HELM>(buf, cpatch, nn, ptab, ctid, kq, tg, kp)
-                               : tri_moments<N, HELM>(buf, sc, cxc, cyc, cdxc, cdyc, da, kq, xg, tg, kp);
-        ssum += mcc;
-        if (room) {
-          double* ne = stack + (size_t)(sp + c) * ES;
-          if (lane == 0) {
-            ne[0] = cxc; ne[1] = cyc; ne[2] = cdxc; ne[3] = cdyc; ne[4] = (double)(level + 1);
-            ne[5] = (double)(ctid >= 0 && ctid < NCACHE_PARENT ? ctid : -1);
-          }
-          if (lane < NP) {
-            ne[6 + lane] = mcc.re;
-            ne[6 + NP + lane] = mcc.im;
-          }
-        }
-      }
+      step_moments<N, HELM>(pb, fb, lsc, false, npar, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
       nref++;
-      // error = max_j |sum children - parent| (complex modulus)
+      nunc += (tid < 0 || tid >= npar) ? 1 : 0;  // children below the cached tree
+      Mom ssum = mc[0];
+      ssum += mc[1];
+      ssum += mc[2];
+      ssum += mc[3];
       const double dre = ssum.re - mpar.re, dim = ssum.im - mpar.im;
       double err = HELM ? sqrt(dre * dre + dim * dim) : fabs(dre);
       err = wave_max(lane < NP ? err : 0.0);
-      if (err > eps && level + 1 <= NLMAX && room) {
+      const bool refine = err > eps;
+      if (refine && lmax < NLMAX && level + 2 > lmax) {
+        // refining would put leaves past the class's level cap: the prediction was wrong
+        aborted = true;
+        break;
+      }
+      if (refine && level + 1 <= NLMAX && sp + 4 <= STACK_CAP) {
+        const double hx = 0.5 * dx, hy = 0.5 * dy;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          double* ne = stack + (size_t)(sp + c) * ES;
+          if (lane == 0) {
+            const int ct = cached_child(tid, c);
+            ne[0] = (c & 1) ? x0 + hx : x0;
+            ne[1] = (c & 2) ? y0 + hy : y0;
+            ne[2] = c == 3 ? -hx : hx;
+            ne[3] = c == 3 ? -hy : hy;
+            ne[4] = (double)(level + 1);
+            ne[5] = (double)(ct >= 0 && ct < NCACHE_PARENT ? ct : -1);
+          }
+          if (lane < NP) {
+            ne[6 + lane] = mc[c].re;
+            ne[6 + NP + lane] = mc[c].im;
+          }
+        }
         sp += 4;
       } else {
-        if (err > eps && level + 1 <= NLMAX && lane == 0) atomicAdd(overflow, 1);
+        if (refine && lane == 0) atomicAdd(overflow, 1);  // level 20 or stack depth reached
         result += ssum;
+        deep = level + 1 > deep ? level + 1 : deep;
       }
       __syncthreads();
     }
-    if (nref_out && lane == 0) nref_out[jt.x] = nref;
-    store_weights<NP, HELM>(buf, result, umat, wnear, iquad[jt.x] - 1);
+    __syncthreads();
+    if (aborted) {
+      if (lane == 0 && fb_out) fb_out[atomicAdd(fb_cnt, 1)] = jt;
+      continue;
+    }
+    if (lane == 0) {
+      if (nref_out) nref_out[jt.x] = nref;
+      if (lev_out) lev_out[jt.x] = deep;
+      if (unc_out) unc_out[jt.x] = nunc;
+    }
+    store_weights<NP, HELM>(pb, result, umat, wnear, iquad[jt.x] - 1);
   }
 }
 
@@ -664,19 +861,15 @@ __global__ void k_features(int nadap, const int2* __restrict__ list, const int64
     for (int k = 0; k < 8; k++) feat_full[8 * (int64_t)jt.x + k] = f[k];
 }
 
-// Bucket key of an adaptive pair (6 npatches keys): predicted classes 0 and 1 (the
-// cached-tree kernels) class-major then by patch; classes 2..5 (the adaptive kernel) by
-// patch, deeper predictions first within a patch.  Without a prediction every pair is in
-// the adaptive part, sorted by patch.
-__device__ __forceinline__ int64_t bucket_key(int c, int64_t patch, int64_t npat) {
-  return c < 2 ? c * npat + patch : 2 * npat + 4 * patch + (5 - c);
-}
+// Bucket key of an adaptive pair (MLP_OUT * npatches keys): rule class, then patch.
+// Without a prediction every pair is in class 0 (the plain rule), sorted by patch.
+__device__ __forceinline__ int64_t bucket_key(int c, int64_t patch, int64_t npat) { return c * npat + patch; }
 
 __global__ void k_bucket_count(int n, const int2* __restrict__ list, const int* __restrict__ pred,
                                const int64_t* __restrict__ col_ind, int64_t npat, int* __restrict__ cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  atomicAdd(cnt + bucket_key(pred ? pred[i] : 5, col_ind[list[i].x] - 1, npat), 1);
+  atomicAdd(cnt + bucket_key(pred ? pred[i] : 0, col_ind[list[i].x] - 1, npat), 1);
 }
 
 __global__ void k_bucket_scatter(int n, const int2* __restrict__ list, const int* __restrict__ pred,
@@ -685,270 +878,9 @@ __global__ void k_bucket_scatter(int n, const int2* __restrict__ list, const int
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int2 jt = list[i];
-  const int64_t key = bucket_key(pred ? pred[i] : 5, col_ind[jt.x] - 1, npat);
+  const int64_t key = bucket_key(pred ? pred[i] : 0, col_ind[jt.x] - 1, npat);
   out[off[key] + atomicAdd(fill + key, 1)] = jt;
   if (pred_full) pred_full[jt.x] = pred[i];
-}
-
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-
-// One 16-pair block of the cached-tree check on the matrix cores.  Lane l evaluates
-// f = K(x_i, y_q) w_q for pair i = l % 16 and node slot l / 16 of a 4-node step; the FP64
-// MFMA 16x16x4 multiplies the 16x4 block of f by the 4x16 block of basis values P_j(y_q)
-// (j = l % 16 + 16 ct) and accumulates the moments of 16 pairs x 16 basis functions;
-// accumulator element r of lane l is pair l / 16 + 4 r, basis function l % 16 + 16 ct.
-template <int N, bool HELM>
-__device__ __forceinline__ void tile_moments(dbl4 (&T)[HELM ? 2 : 1][((N + 1) * (N + 2) / 2 + 15) / 16], int tri,
-                                             int kq, int nn, const double* __restrict__ cp,
-                                             const double* __restrict__ ptab, const double* tg, bool row_ok,
-                                             const KPar& kp) {
-  constexpr int NP = (N + 1) * (N + 2) / 2;
-  constexpr int CT = (NP + 15) / 16;
-  const int lane = threadIdx.x, ia = lane & 15, ka = lane >> 4;
-  // node data of the next step are loaded while the current step is evaluated
-  double g[12], w = 0.0, bv[CT];
-  auto load = [&](int q, double* gg, double& ww, double* bb) {
-#pragma unroll
-    for (int ct = 0; ct < CT; ct++) bb[ct] = 0.0;
-    ww = 0.0;
-    for (int m = 0; m < 12; m++) gg[m] = 0.0;
-    gg[0] = tg[0] + 1.0;
-    gg[1] = tg[1];
-    gg[2] = tg[2];
-    if (q < kq) {
-      const int node = tri * kq + q;
-      const double* cn = cp + (size_t)CACHE_STRIDE * node;
-      gg[0] = cn[0];
-      gg[1] = cn[1];
-      gg[2] = cn[2];
-      gg[9] = cn[3];
-      gg[10] = cn[4];
-      gg[11] = cn[5];
-      ww = row_ok ? cn[6] : 0.0;
-#pragma unroll
-      for (int ct = 0; ct < CT; ct++) {
-        const int j = ia + 16 * ct;
-        if (j < NP) bb[ct] = ptab[(size_t)node * NP + j];
-      }
-    }
-  };
-  // two 4-node steps per iteration: two independent kernel evaluations per lane
-  double g2[12], w2, bv2[CT];
-  load(ka, g, w, bv);
-  load(4 + ka, g2, w2, bv2);
-#pragma unroll 1
-  for (int q0 = 0; q0 < kq; q0 += 8) {
-    double gn[12], wn, bn[CT], gn2[12], wn2, bn2[CT];
-    load(q0 + 8 + ka, gn, wn, bn);
-    load(q0 + 12 + ka, gn2, wn2, bn2);
-    double fr = 0.0, fi = 0.0, fr2 = 0.0, fi2 = 0.0;
-    {
-      double vr, vi, vr2, vi2;
-      kern_eval<HELM>(tg, g, kp, vr, vi);
-      kern_eval<HELM>(tg, g2, kp, vr2, vi2);
-      // w = 0 marks a padding node or row; f must then be an exact zero
-      fr = w != 0.0 ? vr * w : 0.0;
-      fi = w != 0.0 ? vi * w : 0.0;
-      fr2 = w2 != 0.0 ? vr2 * w2 : 0.0;
-      fi2 = w2 != 0.0 ? vi2 * w2 : 0.0;
-    }
-#pragma unroll
-    for (int ct = 0; ct < CT; ct++) {
-      T[0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr, bv[ct], T[0][ct], 0, 0, 0);
-      if (HELM) T[HELM ? 1 : 0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fi, bv[ct], T[HELM ? 1 : 0][ct], 0, 0, 0);
-      T[0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr2, bv2[ct], T[0][ct], 0, 0, 0);
-      if (HELM)
-        T[HELM ? 1 : 0][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(fi2, bv2[ct], T[HELM ? 1 : 0][ct], 0, 0, 0);
-    }
-#pragma unroll
-    for (int m = 0; m < 12; m++) {
-      g[m] = gn[m];
-      g2[m] = gn2[m];
-    }
-    w = wn;
-    w2 = wn2;
-#pragma unroll
-    for (int ct = 0; ct < CT; ct++) {
-      bv[ct] = bn[ct];
-      bv2[ct] = bn2[ct];
-    }
-  }
-}
-
-// per accumulator row r of this lane: max over basis functions of |A - B| (complex
-// modulus), reduced over the 16 lanes that share the row
-template <int NPART, int CT>
-__device__ __forceinline__ void tile_err(const dbl4 (&A)[NPART][CT], const dbl4 (&B)[NPART][CT], double* e) {
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    double m = 0.0;
-#pragma unroll
-    for (int ct = 0; ct < CT; ct++) {
-      const double dr = A[0][ct][r] - B[0][ct][r];
-      double v = dr * dr;
-      if (NPART == 2) {
-        const double di = A[NPART - 1][ct][r] - B[NPART - 1][ct][r];
-        v += di * di;
-      }
-      v = sqrt(v);
-      m = v > m || v != v ? v : m;
-    }
-#pragma unroll
-    for (int off = 8; off >= 1; off >>= 1) {
-      const double o = __shfl_xor(m, off, 64);
-      m = o > m || o != o ? o : m;
-    }
-    e[r] = m;
-  }
-}
-
-// Cached-tree check for pairs whose predicted refinement tree stays inside depth DEPTH
-// (DEPTH 1: the root is not refined; DEPTH 2: only the root is refined).  The tests are
-// the adaptive integrator's own (triaintrouts.f:1435-1448) on the same triangles, so an
-// accepted pair gets the adaptive result; a pair that fails is appended to fb_list and
-// integrated by k_adaptive.
-template <int N, bool HELM, int DEPTH>
-__global__ void __launch_bounds__(64) k_shallow(int npairs, const int2* __restrict__ list, int kq,
-                                                const int64_t* __restrict__ col_ind, int64_t ndtarg,
-                                                const double* __restrict__ targs, const int64_t* __restrict__ iquad,
-                                                const double* __restrict__ epsp_patch,
-                                                const double* __restrict__ umat, KPar kp,
-                                                const double* __restrict__ cache, const double* __restrict__ ptab,
-                                                int* __restrict__ nref_out, int2* __restrict__ fb_list,
-                                                int* __restrict__ counts, double* __restrict__ wnear,
-                                                int chunk) {
-  constexpr int NP = (N + 1) * (N + 2) / 2;
-  constexpr int CT = (NP + 15) / 16;
-  constexpr int NPART = HELM ? 2 : 1;
-  constexpr int RW = NPART * CT * 16;  // staged result row: [part][padded j]
-  __shared__ int code[16];
-  const int lane = threadIdx.x, ia = lane & 15, ka = lane >> 4;
-  // workgroup b runs on XCD b % 8: give each XCD a contiguous run of 16-pair blocks (the
-  // list is sorted by patch) so that a patch's cached geometry is read into one L2
-  const int base = ((blockIdx.x & 7) * chunk + (blockIdx.x >> 3)) * 16;
-  if (base >= npairs) return;
-  const int nn = NCACHE_TRI * kq;
-  const bool row_ok = base + ia < npairs;
-  const int2 jta = list[row_ok ? base + ia : base];
-  const int64_t ipa = col_ind[jta.x] - 1;
-  const double* tp = targs + ndtarg * (int64_t)jta.y;
-  const double tg[3] = {tp[0], tp[1], tp[2]};
-  const double* cp = cache + (size_t)ipa * CACHE_STRIDE * nn;
-  const dbl4 z4 = {0.0, 0.0, 0.0, 0.0};
-  // LDS accumulators (each lane only touches its own elements, so no barrier is needed
-  // until the end): sM = sum of the children, sS = sum of the grandchildren.  The sum of
-  // the current child's children is kept in registers (cS).
-  __shared__ double sM[16 * RW], sS[DEPTH == 2 ? 16 * RW : 1];
-  auto el = [&](int r, int a, int ct) { return (ka + 4 * r) * RW + a * CT * 16 + ct * 16 + ia; };
-  double cmax[4] = {0.0, 0.0, 0.0, 0.0}, e0[4] = {0.0, 0.0, 0.0, 0.0};
-  dbl4 cS[NPART][CT];
-  // triangles in order: DEPTH 1: children 1..4, root; DEPTH 2: the four children of child
-  // c (5 + 4 (c - 1) + g) followed by child c, for c = 1..4, then the root.  One tile call
-  // site keeps the kernel evaluation inlined once.
-  constexpr int NTASK = DEPTH == 1 ? 5 : 21;
-#pragma unroll 1
-  for (int t = 0; t < NTASK; t++) {
-    int tri, c = 0, g = -1;
-    if (t == NTASK - 1) {
-      tri = 0;
-    } else if (DEPTH == 1) {
-      tri = 1 + t;
-      c = 1 + t;
-    } else {
-      c = 1 + t / 5;
-      g = t % 5;  // 0..3: grandchild, 4: the child itself
-      if (g == 4) g = -1;
-      tri = g < 0 ? c : 5 + 4 * (c - 1) + g;
-    }
-    dbl4 T[NPART][CT];
-#pragma unroll
-    for (int a = 0; a < NPART; a++)
-#pragma unroll
-      for (int ct = 0; ct < CT; ct++) T[a][ct] = z4;
-    tile_moments<N, HELM>(T, tri, kq, nn, cp, ptab, tg, row_ok, kp);
-    if (tri == 0) {
-      // root test: the children's sum against the root
-      dbl4 U[NPART][CT];
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int a = 0; a < NPART; a++)
-#pragma unroll
-          for (int ct = 0; ct < CT; ct++) U[a][ct][r] = sM[el(r, a, ct)];
-      tile_err<NPART, CT>(U, T, e0);
-    } else if (g >= 0) {
-      // a grandchild: accumulate the sum of child c's children
-#pragma unroll
-      for (int a = 0; a < NPART; a++)
-#pragma unroll
-        for (int ct = 0; ct < CT; ct++) {
-          if (g == 0) cS[a][ct] = T[a][ct];
-          else
-#pragma unroll
-            for (int r = 0; r < 4; r++) cS[a][ct][r] += T[a][ct][r];
-        }
-    } else {
-      // a child of the root (after its children for DEPTH 2): test it, add it to sM and its
-      // children to sS
-      if (DEPTH == 2) {
-        double ec[4];
-        tile_err<NPART, CT>(cS, T, ec);
-#pragma unroll
-        for (int r = 0; r < 4; r++) cmax[r] = ec[r] > cmax[r] || ec[r] != ec[r] ? ec[r] : cmax[r];
-      }
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int a = 0; a < NPART; a++)
-#pragma unroll
-          for (int ct = 0; ct < CT; ct++) {
-            const int e = el(r, a, ct);
-            sM[e] = (c == 1 ? 0.0 : sM[e]) + T[a][ct][r];
-            if (DEPTH == 2) sS[e] = (c == 1 ? 0.0 : sS[e]) + cS[a][ct][r];
-          }
-    }
-  }
-  // decide per accumulator row; the accepted moments end up in sM
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int i = ka + 4 * r;
-    int cd = -1;  // -1: padding row, 0: send back, 1: accepted at depth 1, 2: depth 2
-    if (base + i < npairs) {
-      const double eps = epsp_patch[col_ind[list[base + i].x] - 1];
-      // refine iff err > eps, as in the adaptive test
-      cd = !(e0[r] > eps) ? 1 : (DEPTH == 2 && !(cmax[r] > eps) ? 2 : 0);
-    }
-    if (ia == 0) code[i] = cd;
-    if (DEPTH == 2 && cd == 2) {
-#pragma unroll
-      for (int a = 0; a < NPART; a++)
-#pragma unroll
-        for (int ct = 0; ct < CT; ct++) sM[el(r, a, ct)] = sS[el(r, a, ct)];
-    }
-  }
-  __syncthreads();
-  // weights = umat^T * moments for the accepted pairs
-  constexpr int NOUT = NPART * NP;
-  for (int o = lane; o < 16 * NOUT; o += 64) {
-    const int i = o / NOUT, rem = o - i * NOUT;
-    if (code[i] <= 0) continue;
-    const int j = rem / NPART, a = rem - j * NPART;
-    const double* m = sM + i * RW + a * CT * 16;
-    double sacc = 0.0;
-    for (int l = 0; l < NP; l++) sacc = fma(umat[l + j * NP], m[l], sacc);
-    const int64_t q0 = iquad[list[base + i].x] - 1;
-    if (HELM) wnear[2 * (q0 + j) + a] = sacc;
-    else wnear[q0 + j] = sacc;
-  }
-  if (lane < 16 && code[lane] >= 0) {
-    const int2 jt = list[base + lane];
-    if (code[lane] == 0) {
-      fb_list[atomicAdd(counts + 4, 1)] = jt;
-    } else {
-      atomicAdd(counts + 4 + code[lane], 1);
-      if (nref_out) nref_out[jt.x] = code[lane] == 1 ? 1 : 5;
-    }
-  }
 }
 
 // ------------------------------------------------------------------ 5. self
@@ -1016,8 +948,13 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
         if (j == lane) pj[j] = Pa[j];
     }
     __syncthreads();
-    for (int q = 0; q < 20; q++)
-      for (int m = 0; m < 3; m++) gx[q / 10][m] += fd[q][m];
+#pragma unroll
+    for (int q = 0; q < 10; q++)
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        gx[0][m] += fd[q][m];
+        gx[1][m] += fd[10 + q][m];
+      }
   }
   double cgl = 0.0;
   for (int i = 0; i < nr; i++) cgl += g_self_gl.w[i] / g_self_gl.x[i];
@@ -1029,9 +966,10 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
   // panel: a 100-node panel would leave 28 lanes of its second pass idle).
   __shared__ double segd[6][8];  // fx, fy, ex, ey, d, ta, tb, npan
   __shared__ int segc[7];        // first node of each segment; segc[nseg] = node count
+  __shared__ int segt[7];        // first ray (tau node) of each segment
   __shared__ double sh_rcut;
   if (lane == 0) {
-    int ns = 0, cnt = 0;
+    int ns = 0, cnt = 0, cntt = 0;
     double hmin = 1e300;
     for (int e = 0; e < 3; e++) {
       const double Ax = vx[e] - u0, Ay = vy[e] - v0;
@@ -1057,12 +995,18 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
         if (!(npan >= 1)) npan = 1;
         double* sd = segd[ns];
         sd[0] = fx; sd[1] = fy; sd[2] = ex; sd[3] = ey; sd[4] = d; sd[5] = ta; sd[6] = tb; sd[7] = npan;
+        segt[ns] = cntt;
         segc[ns++] = cnt;
         cnt += npan * nn;
+        cntt += npan * nt;
       }
     }
     segc[ns] = cnt;
-    for (int k = ns + 1; k < 7; k++) segc[k] = cnt;
+    segt[ns] = cntt;
+    for (int k = ns + 1; k < 7; k++) {
+      segc[k] = cnt;
+      segt[k] = cntt;
+    }
     sh_rcut = 0.5 * hmin;
   }
   __syncthreads();
@@ -1089,14 +1033,6 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
       const double sh = 0.5 * (et - eti), ch = 0.5 * (et + eti);
       const double wtau = g_self_gl.w[it] * (t1 - t0);
       const double px = fx + d * sh * ex, py = fy + d * sh * ey;
-      if (has_dl && ir == 0) {
-        const double dv0 = py / l22, du0 = (px - l12 * dv0) / l11;
-        const double Dx = gx[0][0] * du0 + gx[1][0] * dv0, Dy = gx[0][1] * du0 + gx[1][1] * dv0,
-                     Dz = gx[0][2] * du0 + gx[1][2] * dv0;
-        const double dn = Dx * g0[9] + Dy * g0[10] + Dz * g0[11], d2 = Dx * Dx + Dy * Dy + Dz * Dz;
-        const double kap = -dn / (4.0 * M_PI * d2 * sqrt(d2));
-        pvs += wtau * d * d * ch / detl * kap * (log(d * ch / rcut) - cgl);
-      }
       const double rho = g_self_gl.x[ir];
       const double xi0 = rho * px, xi1 = rho * py;
       const double dv = xi1 / l22, du = (xi0 - l12 * dv) / l11;
@@ -1113,6 +1049,29 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
     acc += column_sum<NP, HELM>(buf, nrows);
   }
   if (has_dl) {
+    // the disk correction of every ray (tau node): rays spread over the lanes
+    const int ntau = segt[6];
+    for (int k = lane; k < ntau; k += WAVE) {
+      int sgi = 0;
+      while (sgi < 5 && k >= segt[sgi + 1]) sgi++;
+      const double* sd = segd[sgi];
+      const double fx = sd[0], fy = sd[1], ex = sd[2], ey = sd[3], d = sd[4], ta = sd[5], tb = sd[6];
+      const int npan = (int)sd[7];
+      const int loc = k - segt[sgi];
+      const int ipan = loc / nt, it = loc - ipan * nt;
+      const double t0 = ta + (tb - ta) * ipan / npan, t1 = ta + (tb - ta) * (ipan + 1) / npan;
+      const double tau = t0 + (t1 - t0) * g_self_gl.x[it];
+      const double et = exp(tau), eti = 1.0 / et;
+      const double sh = 0.5 * (et - eti), ch = 0.5 * (et + eti);
+      const double wtau = g_self_gl.w[it] * (t1 - t0);
+      const double px = fx + d * sh * ex, py = fy + d * sh * ey;
+      const double dv0 = py / l22, du0 = (px - l12 * dv0) / l11;
+      const double Dx = gx[0][0] * du0 + gx[1][0] * dv0, Dy = gx[0][1] * du0 + gx[1][1] * dv0,
+                   Dz = gx[0][2] * du0 + gx[1][2] * dv0;
+      const double dn = Dx * g0[9] + Dy * g0[10] + Dz * g0[11], d2 = Dx * Dx + Dy * Dy + Dz * Dz;
+      const double kap = -dn / (4.0 * M_PI * d2 * sqrt(d2));
+      pvs += wtau * d * d * ch / detl * kap * (log(d * ch / rcut) - cgl);
+    }
     // beta * (sum over rays) * |J0| * P_j(uv0) joins moment j (lane j)
     const double pv = wave_sum(pvs);
     const double jac0 = sqrt((g0[4] * g0[8] - g0[5] * g0[7]) * (g0[4] * g0[8] - g0[5] * g0[7]) +
@@ -1140,7 +1099,9 @@ T* dalloc(size_t n) {
 // so a buffer is never resized while a kernel of an earlier call uses it.
 enum WsSlot {
   W_CMS, W_RADS, W_EPSP, W_AREA, W_CNT, W_OFF, W_FILL, W_CLS, W_COUNTS, W_ADAP, W_SELF, W_FIXED, W_NREF,
-  W_CACHE, W_STACK, W_X, W_PRED, W_BCNT, W_BOFF, W_SORTED, W_FB, W_PFULL, W_QCTR, W_NSLOTS
+  W_CACHE, W_STACK, W_X, W_PRED, W_BCNT, W_BOFF, W_SORTED, W_FB, W_PFULL, W_QCTR, W_LEV, W_SEG, W_UNC,
+  W_RCACHE0,  // tree caches of the routed rule classes: W_RCACHE0 + class
+  W_NSLOTS = W_RCACHE0 + MAX_RULES
 };
 struct WsBuf {
   void* p = nullptr;
@@ -1176,6 +1137,14 @@ struct OrderTables {
   double *umat, *rvw, *rvuv, *xg, *fuv, *Pq, *Bq, *tuv, *twq, *ptab;
 };
 std::map<std::tuple<int, int, int, int, int>, OrderTables> g_tables;
+
+// The adaptive integrator's XG rule of degree q with the node tables of the cached tree
+// (depths 0..3) for patches of order N; built once per device.
+struct RuleTables {
+  int kq = 0;
+  double *xg, *tuv, *twq, *ptab;
+};
+std::map<std::tuple<int, int, int>, RuleTables> g_rules;
 
 #define HCHK(x)                                                                   \
   do {                                                                            \
@@ -1243,6 +1212,44 @@ T* upload(const std::vector<T>& h, hipStream_t s) {
   return d;
 }
 
+// Node tables of the cached tree (depths 0..3: 85 triangles) for the rule (xq, xw):
+// (u, v) and area-scaled weight of every node and the basis values there (patch
+// independent).
+template <int N>
+void cached_tree_tables(const std::vector<double>& xq, const std::vector<double>& xw, std::vector<double>& tuv,
+                        std::vector<double>& twq, std::vector<double>& ptab) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  const int kq = (int)xw.size();
+  const int ncn = NCACHE_TRI * kq;
+  tuv.assign(2 * ncn, 0.0);
+  twq.assign(ncn, 0.0);
+  // 64 rows of padding: the adaptive kernel copies whole 64-row blocks of the table
+  ptab.assign((size_t)(ncn + WAVE) * ptab_w(NP), 0.0);
+  std::vector<double> pols(NP);
+  struct Tri { double x0, y0, dx, dy; };
+  std::vector<Tri> tris(NCACHE_TRI);
+  tris[0] = {0.0, 0.0, 1.0, 1.0};
+  auto kids = [](Tri t, int c) -> Tri {
+    const double hx = t.dx / 2, hy = t.dy / 2;
+    const Tri k[4] = {{t.x0, t.y0, hx, hy}, {t.x0 + hx, t.y0, hx, hy}, {t.x0, t.y0 + hy, hx, hy},
+                      {t.x0 + hx, t.y0 + hy, -hx, -hy}};
+    return k[c];
+  };
+  for (int t = 0; t < NCACHE_PARENT; t++)
+    for (int c = 0; c < 4; c++) tris[cached_child(t, c)] = kids(tris[t], c);
+  for (int t = 0; t < NCACHE_TRI; t++) {
+    const double da = t == 0 ? 1.0 : (t <= 4 ? 0.25 : (t <= 20 ? 0.0625 : 0.015625));
+    for (int q = 0; q < kq; q++) {
+      const int n = t * kq + q;
+      tuv[2 * n] = tris[t].x0 + tris[t].dx * xq[2 * q];
+      tuv[2 * n + 1] = tris[t].y0 + tris[t].dy * xq[2 * q + 1];
+      twq[n] = da * xw[q];
+      fmmhost::koorn_pols(tuv[2 * n], tuv[2 * n + 1], N, pols.data());
+      for (int l = 0; l < NP; l++) ptab[(size_t)n * ptab_w(NP) + l] = pols[l];
+    }
+  }
+}
+
 // umat, RV nodes/weights, the XG rule of the adaptive integrator, the fixed rule with its
 // basis (Pq) and weight-folded (Bq) tables, and the node tables of the cached tree
 template <int N>
@@ -1265,33 +1272,8 @@ int build_tables(int nqorder, int nlev, int nqorder_f, OrderTables& T, hipStream
       Bq[(size_t)q * NP + j] = fw[q] * sacc;
     }
   }
-  // node tables of the cached tree depths 0..3 (patch independent)
-  const int ncn = NCACHE_TRI * kq;
-  std::vector<double> tuv(2 * ncn), twq(ncn), ptab((size_t)ncn * NP);
-  {
-    struct Tri { double x0, y0, dx, dy; };
-    std::vector<Tri> tris(NCACHE_TRI);
-    tris[0] = {0.0, 0.0, 1.0, 1.0};
-    auto kids = [](Tri t, int c) -> Tri {
-      const double hx = t.dx / 2, hy = t.dy / 2;
-      const Tri k[4] = {{t.x0, t.y0, hx, hy}, {t.x0 + hx, t.y0, hx, hy}, {t.x0, t.y0 + hy, hx, hy},
-                        {t.x0 + hx, t.y0 + hy, -hx, -hy}};
-      return k[c];
-    };
-    for (int t = 0; t < NCACHE_PARENT; t++)
-      for (int c = 0; c < 4; c++) tris[cached_child(t, c)] = kids(tris[t], c);
-    for (int t = 0; t < NCACHE_TRI; t++) {
-      const double da = t == 0 ? 1.0 : (t <= 4 ? 0.25 : (t <= 20 ? 0.0625 : 0.015625));
-      for (int q = 0; q < kq; q++) {
-        const int n = t * kq + q;
-        tuv[2 * n] = tris[t].x0 + tris[t].dx * xq[2 * q];
-        tuv[2 * n + 1] = tris[t].y0 + tris[t].dy * xq[2 * q + 1];
-        twq[n] = da * xw[q];
-        fmmhost::koorn_pols(tuv[2 * n], tuv[2 * n + 1], N, pols.data());
-        for (int l = 0; l < NP; l++) ptab[(size_t)n * NP + l] = pols[l];
-      }
-    }
-  }
+  std::vector<double> tuv, twq, ptab;
+  cached_tree_tables<N>(xq, xw, tuv, twq, ptab);
   T.kq = kq;
   T.nf = nf;
   T.umat = upload(umat, s);
@@ -1307,6 +1289,48 @@ int build_tables(int nqorder, int nlev, int nqorder_f, OrderTables& T, hipStream
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
   if (!T.umat || !T.rvw || !T.rvuv || !T.xg || !T.fuv || !T.Pq || !T.Bq || !T.tuv || !T.twq || !T.ptab) return -2;
   return 0;
+}
+
+// tables of the adaptive rule of degree q for order N (routed classes)
+template <int N>
+int rule_tables(int q, const RuleTables** out, hipStream_t s) {
+  int dev = 0;
+  HCHK(hipGetDevice(&dev));
+  const auto key = std::make_tuple(dev, N, q);
+  auto it = g_rules.find(key);
+  if (it == g_rules.end()) {
+    std::vector<double> xq, xw;
+    const int kq = fmmhost::xg_rule(q, xq, xw);
+    std::vector<double> xg(3 * kq), tuv, twq, ptab;
+    for (int k = 0; k < kq; k++) { xg[3 * k] = xq[2 * k]; xg[3 * k + 1] = xq[2 * k + 1]; xg[3 * k + 2] = xw[k]; }
+    cached_tree_tables<N>(xq, xw, tuv, twq, ptab);
+    RuleTables T;
+    T.kq = kq;
+    T.xg = upload(xg, s);
+    T.tuv = upload(tuv, s);
+    T.twq = upload(twq, s);
+    T.ptab = upload(ptab, s);
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    if (!T.xg || !T.tuv || !T.twq || !T.ptab) return -2;
+    it = g_rules.emplace(key, T).first;
+  }
+  *out = &it->second;
+  return 0;
+}
+
+// XG degree of rule class c around the plain degree q0 (class 2 is the plain rule); no
+// class goes below degree 8, where the children-against-parent test stops being a
+// reliable error estimate, or above 30, the largest rule in the tables
+inline int class_degree(int q0, int c) {
+  static const int off[MAX_RULES] = {-4, -2, 0, 4, 8, 12};
+  const int q = q0 + off[c];
+  return q < 8 ? (q0 < 8 ? q0 : 8) : (q > 30 ? 30 : q);
+}
+constexpr int PLAIN_CLASS = 2;
+
+__global__ void k_seg_bounds(int nrules, int64_t npat, const int* __restrict__ boff, int* __restrict__ seg) {
+  const int c = threadIdx.x;
+  if (c <= nrules) seg[c] = boff[c * npat];
 }
 
 // Persistent wavefronts of the adaptive kernel: one per resident slot on this device
@@ -1329,6 +1353,15 @@ int adaptive_slots() {
   if (n > 8192) n = 8192;
   slots[dev] = n;
   return n;
+}
+
+template <int N, bool HELM>
+void launch_adaptive(int g, hipStream_t s, int nlist, const int* nlist_dev, const int2* list, const AdapRules& R,
+                     int2* fb_out, int* fb_cnt, int* qctr, const NearQuadArgs& A, const double* epsp,
+                     const double* umat, const KPar& kp, double* stack, int* nref, int* lev, int* unc, int* over) {
+  k_adaptive<N, HELM><<<g, 64, 0, s>>>(nlist, nlist_dev, list, R, fb_out, fb_cnt, qctr, A.col_ind, A.ixyzs,
+                                       A.srccoefs, A.ndtarg, A.targs, A.iquad, epsp, umat, kp, stack, nref, lev, unc,
+                                       over, A.wnear);
 }
 
 template <int N, bool HELM>
@@ -1412,7 +1445,8 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     HCHK(hipEventRecord(ev[14], s));
     HCHK(hipStreamWaitEvent(s2, ev[14], 0));
     HCHK(hipEventRecord(ev[9], s2));
-    k_tree_cache<N><<<npat, 256, 0, s2>>>(npat, kq, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs, d_cache);
+    k_tree_cache<N><<<npat, 256, 0, s2>>>(npat, kq, NCACHE_TRI, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs,
+                                          d_cache);
     HCHK(hipGetLastError());
     HCHK(hipEventRecord(ev[8], s2));
   }
@@ -1430,9 +1464,15 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     return -3;
   }
   if (s2 != s) HCHK(hipStreamWaitEvent(s2, ev[1], 0));
-  const size_t lds_fixed = sizeof(double) * (9 * NP + 7 * (size_t)nf);
-  k_fixed<N, HELM><<<npat, 128, lds_fixed, s2>>>(nf, d_fuv, d_Pq, d_Bq, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
-                                                 A.iquad, d_off, d_fixed, kp, A.wnear);
+  if (nf % 4 == 0) {
+    const size_t lds_fixed = sizeof(double) * (9 * NP + 8 * (size_t)nf);
+    k_fixed_mfma<N, HELM><<<npat, 256, lds_fixed, s2>>>(nf, d_Pq, d_Bq, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
+                                                        A.iquad, d_off, d_fixed, kp, A.wnear);
+  } else {
+    const size_t lds_fixed = sizeof(double) * (9 * NP + 7 * (size_t)nf);
+    k_fixed<N, HELM><<<npat, 128, lds_fixed, s2>>>(nf, d_fuv, d_Pq, d_Bq, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
+                                                   A.iquad, d_off, d_fixed, kp, A.wnear);
+  }
   HCHK(hipGetLastError());
   HCHK(hipEventRecord(ev[2], s2));
   const bool self_side = s2 != s;
@@ -1464,29 +1504,32 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   double* d_stack = wsalloc<double>(W_STACK, (size_t)nwaves * STACK_CAP * ES);
   if (!d_stack) return -2;
   int* d_over = d_counts + 2;
-  // Adaptive pairs are bucketed by patch (and, with a selector, by predicted depth) so
-  // that consecutive work items share a patch's cached geometry.
-  const bool learned = A.mlp_params != nullptr && N <= 6 && nadap > 0;
+  // Adaptive pairs are bucketed by patch (and, with a selector, by predicted rule class
+  // first) so that consecutive work items share a patch's cached geometry.
+  const bool learned = A.mlp_params != nullptr && nadap > 0;
   const bool want_feat = (learned || A.feat_out) && nadap > 0;
   float* d_X = want_feat ? wsalloc<float>(W_X, (size_t)nadap * 8) : nullptr;
   int *d_pred = nullptr, *d_boff = nullptr, *d_bcnt = nullptr, *d_pfull = nullptr, *d_qctr = nullptr;
+  int *d_lev = nullptr, *d_seg = nullptr, *d_unc = nullptr;
   int2 *d_sorted = nullptr, *d_fb = nullptr;
   if (want_feat && !d_X) return -2;
   if (nadap > 0) {
     d_bcnt = wsalloc<int>(W_BCNT, MLP_OUT * npat);
     d_boff = wsalloc<int>(W_BOFF, MLP_OUT * npat + 1);
     d_sorted = wsalloc<int2>(W_SORTED, nadap);
-    d_qctr = wsalloc<int>(W_QCTR, 9);
-    if (!d_bcnt || !d_boff || !d_sorted || !d_qctr) return -2;
+    d_qctr = wsalloc<int>(W_QCTR, 16);
+    if (A.lev_host) d_lev = wsalloc<int>(W_LEV, A.nnz);
+    if (A.unc_host) d_unc = wsalloc<int>(W_UNC, A.nnz);
+    if (!d_bcnt || !d_boff || !d_sorted || !d_qctr || (A.lev_host && !d_lev) || (A.unc_host && !d_unc)) return -2;
   }
   if (learned) {
     d_pred = wsalloc<int>(W_PRED, nadap);
     d_fb = wsalloc<int2>(W_FB, nadap);
+    d_seg = wsalloc<int>(W_SEG, MAX_RULES + 1);
     if (A.pred_host) d_pfull = wsalloc<int>(W_PFULL, A.nnz);
-    if (!d_pred || !d_fb || (A.pred_host && !d_pfull)) return -2;
+    if (!d_pred || !d_fb || !d_seg || (A.pred_host && !d_pfull)) return -2;
   }
   HCHK(hipEventRecord(ev[5], s));
-  int nb[3] = {0, 0, nadap};  // pairs routed to depth 1, depth 2, the adaptive kernel
   if (want_feat) {
     const double kabs = HELM ? sqrt(A.zk[0] * A.zk[0] + A.zk[1] * A.zk[1]) : 0.0;
     k_features<N><<<(nadap + tb - 1) / tb, tb, 0, s>>>(nadap, d_adap, A.col_ind, A.ixyzs, A.srcvals, A.ndtarg,
@@ -1498,9 +1541,11 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     if (mlp_predict_device(A.mlp_params, d_X, nadap, d_pred, nullptr, s) != 0) return -1;
     if (d_pfull) HCHK(hipMemsetAsync(d_pfull, 0xff, sizeof(int) * A.nnz, s));
   }
+  if (d_lev) HCHK(hipMemsetAsync(d_lev, 0, sizeof(int) * A.nnz, s));
+  if (d_unc) HCHK(hipMemsetAsync(d_unc, 0, sizeof(int) * A.nnz, s));
   if (nadap > 0) {
     HCHK(hipMemsetAsync(d_bcnt, 0, sizeof(int) * MLP_OUT * npat, s));
-    HCHK(hipMemsetAsync(d_qctr, 0, sizeof(int) * 9, s));
+    HCHK(hipMemsetAsync(d_qctr, 0, sizeof(int) * 16, s));
     k_bucket_count<<<(nadap + tb - 1) / tb, tb, 0, s>>>(nadap, d_adap, d_pred, A.col_ind, npat, d_bcnt);
     k_scan<<<1, 1024, 0, s>>>(MLP_OUT * npat, d_bcnt, d_boff);
     HCHK(hipMemsetAsync(d_bcnt, 0, sizeof(int) * MLP_OUT * npat, s));
@@ -1508,14 +1553,27 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
                                                           d_sorted, d_pfull);
     HCHK(hipGetLastError());
   }
+  // rule classes: the plain rule alone, or with a selector every class (class c: XG
+  // degree class_degree(nqorder, c), level cap from route_lmax) whose segment of the
+  // sorted list is not empty
+  AdapRules R = {};
+  int seg[MAX_RULES + 1] = {0};
+  int nroute = 0;  // routed classes with pairs
   if (learned) {
-    int b1 = 0, b2 = 0;
-    HCHK(hipMemcpyAsync(&b1, d_boff + npat, sizeof(int), hipMemcpyDeviceToHost, s));
-    HCHK(hipMemcpyAsync(&b2, d_boff + 2 * npat, sizeof(int), hipMemcpyDeviceToHost, s));
+    k_seg_bounds<<<1, 64, 0, s>>>(MAX_RULES, npat, d_boff, d_seg);
+    HCHK(hipMemcpyAsync(seg, d_seg, sizeof(int) * (MAX_RULES + 1), hipMemcpyDeviceToHost, s));
     HCHK(hipStreamSynchronize(s));
-    nb[0] = b1;
-    nb[1] = b2 - b1;
-    nb[2] = nadap - b2;
+    R.nrules = MAX_RULES;
+    for (int c = 0; c <= MAX_RULES; c++) R.seg[c] = seg[c];
+  } else {
+    R.nrules = 1;
+    R.seg[0] = 0;
+    R.seg[1] = nadap;
+    R.kq[0] = kq;
+    R.lmax[0] = NLMAX;
+    R.npar[0] = NCACHE_PARENT;
+    R.xg[0] = d_xg;
+    R.ptab[0] = d_ptab;
   }
   HCHK(hipEventRecord(ev[6], s));
   if (nadap > 0) {
@@ -1523,30 +1581,65 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
       HCHK(hipStreamWaitEvent(s, ev[8], 0));
     } else {
       HCHK(hipEventRecord(ev[9], s));
-      k_tree_cache<N><<<npat, 256, 0, s>>>(npat, kq, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs, d_cache);
+      k_tree_cache<N><<<npat, 256, 0, s>>>(npat, kq, NCACHE_TRI, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs,
+                                           d_cache);
       HCHK(hipEventRecord(ev[8], s));
     }
     HCHK(hipEventRecord(ev[11], s));
     if (learned) {
-      // grid = 8 x chunk workgroups, chunk = 16-pair blocks per XCD
-      const int c0 = ((nb[0] + 15) / 16 + 7) / 8, c1 = ((nb[1] + 15) / 16 + 7) / 8;
-      if (nb[0] > 0)
-        k_shallow<N, HELM, 1><<<8 * c0, 64, 0, s>>>(nb[0], d_sorted, kq, A.col_ind, A.ndtarg, A.targs, A.iquad,
-                                                    d_epsp, d_umat, kp, d_cache, d_ptab, d_nref, d_fb, d_counts,
-                                                    A.wnear, c0);
-      if (nb[1] > 0)
-        k_shallow<N, HELM, 2><<<8 * c1, 64, 0, s>>>(nb[1], d_sorted + nb[0], kq, A.col_ind, A.ndtarg, A.targs,
-                                                    A.iquad, d_epsp, d_umat, kp, d_cache, d_ptab, d_nref, d_fb,
-                                                    d_counts, A.wnear, c1);
+      // tree caches of the routed classes (the plain class shares the plain cache)
+      for (int c = 0; c < MAX_RULES; c++) {
+        const int q = class_degree(nqorder, c);
+        const RuleTables* RT = nullptr;
+        if (rule_tables<N>(q, &RT, s) != 0) return -2;
+        R.kq[c] = RT->kq;
+        R.xg[c] = RT->xg;
+        R.ptab[c] = RT->ptab;
+        R.lmax[c] = A.route_lmax ? A.route_lmax[c] : NLMAX;
+        if (R.lmax[c] < 1 || R.lmax[c] > NLMAX) R.lmax[c] = NLMAX;
+        R.cache[c] = nullptr;
+        if (seg[c + 1] == seg[c]) continue;
+        nroute++;
+        if (q == nqorder) {
+          R.npar[c] = NCACHE_PARENT;
+          R.cache[c] = d_cache;
+          continue;
+        }
+        // the class's pairs evaluate no triangle below depth lmax (deeper ones go back to
+        // the plain integrator), so its cache holds depths 0 .. min(3, lmax)
+        R.npar[c] = cache_parents(R.lmax[c]);
+        const int ntri = cache_tris(R.npar[c]);
+        double* cc = wsalloc<double>(W_RCACHE0 + c, (size_t)npat * CACHE_STRIDE * ntri * RT->kq);
+        if (!cc) return -2;
+        k_tree_cache<N><<<npat, 256, 0, s>>>(npat, RT->kq, ntri, RT->tuv, RT->twq, RT->ptab, A.ixyzs, A.srccoefs,
+                                             cc);
+        R.cache[c] = cc;
+      }
       HCHK(hipGetLastError());
+    } else {
+      R.cache[0] = d_cache;
     }
     HCHK(hipEventRecord(ev[7], s));
     const int g = nadap < nwaves ? nadap : nwaves;
-    k_adaptive<N, HELM><<<g, 64, 0, s>>>(nb[2], d_sorted + nb[0] + nb[1], d_fb, learned ? d_counts + 4 : nullptr,
-                                         d_qctr, kq, d_xg, A.col_ind, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
-                                         A.iquad, d_epsp, d_umat, kp, d_stack, d_nref, d_over, A.wnear, d_cache,
-                                         d_ptab);
+    launch_adaptive<N, HELM>(g, s, nadap, nullptr, d_sorted, R, d_fb, d_counts + 4, d_qctr, A, d_epsp, d_umat,
+                             kp, d_stack, d_nref, d_lev, d_unc, d_over);
     HCHK(hipGetLastError());
+    if (learned) {
+      // pairs whose tree went past their class's level: the plain rule and level cap
+      AdapRules P0 = {};
+      P0.nrules = 1;
+      P0.seg[0] = 0;
+      P0.seg[1] = 0x7fffffff;
+      P0.kq[0] = kq;
+      P0.lmax[0] = NLMAX;
+      P0.npar[0] = NCACHE_PARENT;
+      P0.xg[0] = d_xg;
+      P0.ptab[0] = d_ptab;
+      P0.cache[0] = d_cache;
+      launch_adaptive<N, HELM>(g, s, 0, d_counts + 4, d_fb, P0, nullptr, nullptr, d_qctr + 8, A, d_epsp,
+                               d_umat, kp, d_stack, d_nref, d_lev, d_unc, d_over);
+      HCHK(hipGetLastError());
+    }
   } else {
     if (!early_tree) {
       HCHK(hipEventRecord(ev[9], s));
@@ -1563,6 +1656,8 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   HCHK(hipStreamSynchronize(s));
   for (int i = 0; i < 4; i++) counts[i] = counts8[i];
   if (d_pfull) HCHK(hipMemcpy(A.pred_host, d_pfull, sizeof(int) * A.nnz, hipMemcpyDeviceToHost));
+  if (d_lev) HCHK(hipMemcpy(A.lev_host, d_lev, sizeof(int) * A.nnz, hipMemcpyDeviceToHost));
+  if (d_unc) HCHK(hipMemcpy(A.unc_host, d_unc, sizeof(int) * A.nnz, hipMemcpyDeviceToHost));
   if (st) {
     st->nself = nself;
     st->nadap = nadap;
@@ -1583,9 +1678,9 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     st->ms_shallow = elapsed(11, 7);
     st->ms_adap = elapsed(7, 3);
     st->ms_self = elapsed(12, 13);
-    st->nshallow0 = counts8[5];
-    st->nshallow1 = counts8[6];
+    st->nroute = nroute;
     st->nfallback = counts8[4];
+    for (int c = 0; c < MAX_RULES; c++) st->ncls[c] = learned ? seg[c + 1] - seg[c] : (c == 0 ? nadap : 0);
     st->nref_total = 0;
     if (d_nref && A.nref_host) HCHK(hipMemcpy(A.nref_host, d_nref, sizeof(int) * A.nnz, hipMemcpyDeviceToHost));
     if (A.class_host) HCHK(hipMemcpy(A.class_host, d_cls, sizeof(int) * A.nnz, hipMemcpyDeviceToHost));

@@ -29,22 +29,24 @@ struct NearQuadArgs {
   bool want_nref;
   int* nref_host;          // optional: refinements per pair (nnz)
   int* class_host;         // optional: pair class (nnz): 0 self, 1 adaptive, 2 fixed
-  // learned refinement-depth selector (orders 1..6): MLP parameters in HBM with the
-  // feature normalisation folded into the first layer; null = plain adaptive path
+  // learned order selection: MLP parameters in HBM (feature normalisation folded into
+  // the first layer) predicting the rule class of every adaptive pair; null = plain path
   const float* mlp_params;
   float* feat_out;         // optional, device (nnz, 8): selector features of adaptive pairs
-  int* pred_host;          // optional: predicted depth class per pair (-1 if not adaptive)
+  int* pred_host;          // optional: predicted rule class per pair (-1 if not adaptive)
+  const int* route_lmax;   // optional, host (6): deepest level of each rule class (default 20)
+  int* lev_host;           // optional: deepest level of every adaptive pair's tree (nnz)
+  int* unc_host;           // optional: refinement tests below the cached tree per pair (nnz)
 };
 
 struct NearQuadStats {
   int nself, nadap, nfixed, overflow, nf, kq;
   double ms_prep, ms_fixed, ms_adap, ms_self;
   long long nref_total;
-  // learned path: pairs accepted by the depth-1 / depth-2 cached-tree kernels and pairs
-  // sent back to the adaptive kernel.  Times: host tables + uploads, features + inference
-  // + bucketing, depth-0..2 geometry cache, cached-tree kernels; ms_adap is the adaptive
-  // kernel alone.
-  int nshallow0, nshallow1, nfallback;
+  // learned path: rule classes with pairs, pairs per class, pairs sent back to the plain
+  // integrator.  Times: host tables + uploads, features + inference + bucketing, the plain
+  // tree cache, the routed classes' tree caches; ms_adap is the adaptive kernel launches.
+  int nroute, nfallback, ncls[6];
   double ms_tables, ms_select, ms_tree, ms_shallow;
 };
 

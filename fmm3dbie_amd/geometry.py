@@ -46,25 +46,20 @@ def _finish(skel_eval, npatches, norder):
 
 
 def _rectmesh(v1, v2, v3, v4, nu, nv):
-    """xtri_rectmesh_3d: 2*nu*nv flat triangles (3 vertices each) of a quadrilateral."""
-    out = []
-    vl = v4 - v1
-    vr = v3 - v2
-    for i in range(1, nv + 1):
-        uvw1 = v1 + (i - 1) * vl / nv
-        uvw4 = uvw1 + vl / nv
-        vb = v2 + (i - 1) * vr / nv - uvw1
-        vt = uvw1 + vb + vr / nv - uvw4
-        uvw2 = uvw1 + vb / nu
-        uvw3 = uvw4 + vt / nu
-        for _ in range(nu):
-            out.append((uvw1.copy(), uvw2.copy(), uvw4.copy()))
-            out.append((uvw3.copy(), uvw4.copy(), uvw2.copy()))
-            uvw1 = uvw2.copy()
-            uvw2 = uvw2 + vb / nu
-            uvw4 = uvw3.copy()
-            uvw3 = uvw3 + vt / nu
-    return out
+    """2 nu nv flat triangles of the quadrilateral v1 v2 v3 v4 (xtri_rectmesh_3d,
+    xtri_parameterizations.f90:1020-1077): grid points X(s, t) = (1 - t)((1 - s) v1 + s v2)
+    + t((1 - s) v4 + s v3) at s = j / nu, t = i / nv; cell (i, j), rows i outer and
+    columns j inner, gives (X00, X10, X01) then (X11, X01, X10) -- the reference's
+    triangle order, which fixes the order of the surface's points."""
+    v1, v2, v3, v4 = (np.asarray(v, np.float64) for v in (v1, v2, v3, v4))
+    s = np.arange(nu + 1) / nu
+    t = np.arange(nv + 1) / nv
+    S, T = s[None, :, None], t[:, None, None]
+    X = (1 - T) * ((1 - S) * v1 + S * v2) + T * ((1 - S) * v4 + S * v3)  # (nv+1, nu+1, 3)
+    x00, x10, x01, x11 = X[:-1, :-1], X[:-1, 1:], X[1:, :-1], X[1:, 1:]
+    ta = np.stack([x00, x10, x01], axis=2)  # (nv, nu, 3 vertices, 3)
+    tb = np.stack([x11, x01, x10], axis=2)
+    return list(np.stack([ta, tb], axis=2).reshape(-1, 3, 3))
 
 
 def cube_skeleton(a, b, c, na, nb, nc):

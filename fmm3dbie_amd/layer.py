@@ -277,7 +277,17 @@ def set_side_stream(enable=True):
     """Near-quadrature concurrency: True (default) runs the fixed-rule and self kernels on
     a second stream beside the adaptive kernels; False keeps every kernel in order on the
     caller's stream, so that per-kernel times are those of the kernel alone."""
+    global _SIDE
+    _SIDE = bool(enable)
     lib().fmm3dbie_set_side_stream(int(bool(enable)))
+
+
+_SIDE = True
+
+
+def side_stream_enabled():
+    """The current side-stream setting (set_side_stream; on by default)."""
+    return _SIDE
 
 
 class NearQuadDevice:
@@ -321,16 +331,21 @@ class NearQuadDevice:
         self.wnear = torch.zeros(self.nquad, dtype=torch.complex128 if self.helm else torch.float64, device=device)
         self.stats = None
 
-    def run(self, stream=None, want_detail=False, selector=None, features=None):
+    def run(self, stream=None, want_detail=False, selector=None, features=None, route_lmax=None):
         """Fill self.wnear.  selector: device float32 tensor of MLP parameters (learned
-        refinement-depth routing, orders 1..6), or None for the plain adaptive path.
-        features: device float32 tensor (nnz, 8) that receives the selector features of
-        the adaptive pairs.  want_detail adds per-pair class, refinement count and (with a
-        selector) predicted depth class to self.stats."""
+        order selection: every adaptive pair integrated with the rule class the network
+        predicts), or None for the plain adaptive path.  route_lmax: the deepest level of
+        each of the 6 rule classes (a pair whose tree needs more is integrated again by
+        the plain integrator); default 20.  features: device float32 tensor (nnz, 8) that
+        receives the selector features of the adaptive pairs.  want_detail adds per-pair
+        class, refinement-test count, deepest level, tests below the cached tree and
+        (with a selector) predicted rule class to self.stats."""
         torch = _torch()
-        st = np.zeros(20)
+        st = np.zeros(24)
         nref = np.zeros(self.nnz, np.int32) if want_detail else None
         cls = np.zeros(self.nnz, np.int32) if want_detail else None
+        lev = np.zeros(self.nnz, np.int32) if want_detail else None
+        unc = np.zeros(self.nnz, np.int32) if want_detail else None
         pred = np.zeros(self.nnz, np.int32) if (want_detail and selector is not None) else None
         if selector is not None and (selector.dtype != torch.float32 or not selector.is_cuda
                                      or selector.numel() != lib().fmm3dbie_mlp_nparam()):
@@ -339,6 +354,11 @@ class NearQuadDevice:
         if features is not None and (features.dtype != torch.float32 or not features.is_cuda
                                      or tuple(features.shape) != (self.nnz, 8) or not features.is_contiguous()):
             raise ValueError("features must be a contiguous device float32 tensor of shape (nnz, 8)")
+        lm = None
+        if route_lmax is not None:
+            lm = np.ascontiguousarray(route_lmax, dtype=np.int32)
+            if lm.shape != (6,):
+                raise ValueError("route_lmax must hold 6 levels")
         s = stream if stream is not None else torch.cuda.current_stream()
         rc = lib().fmm3dbie_nearquad_dev(
             int(self.helm), self.norder, self.norder_branch, self.npatches, _dptr(self.d_ixyzs),
@@ -348,16 +368,19 @@ class NearQuadDevice:
             _dptr(self.d_row), _dptr(self.d_col), _dptr(self.d_iquad), self.rfac0, _dptr(self.wnear), 10, 1.0,
             C.c_void_p(s.cuda_stream), _p(st), _p(nref) if want_detail else None, _p(cls) if want_detail else None,
             _dptr(selector) if selector is not None else None, _dptr(features) if features is not None else None,
-            _p(pred) if pred is not None else None)
+            _p(pred) if pred is not None else None, _p(lm) if lm is not None else None,
+            _p(lev) if want_detail else None, _p(unc) if want_detail else None)
         if rc != 0:
             raise RuntimeError("fmm3dbie_nearquad_dev failed (status %d)" % rc)
         self.stats = dict(nself=int(st[0]), nadap=int(st[1]), nfixed=int(st[2]), overflow=int(st[3]), nf=int(st[4]),
                           kq=int(st[5]), ms_prep=st[6], ms_fixed=st[7], ms_adap=st[8], ms_self=st[9],
-                          ms_tables=st[10], ms_select=st[11], ms_tree=st[12], ms_shallow=st[13],
-                          nshallow0=int(st[14]), nshallow1=int(st[15]), nfallback=int(st[16]))
+                          ms_tables=st[10], ms_select=st[11], ms_tree=st[12], ms_tree_routed=st[13],
+                          nroute=int(st[14]), nfallback=int(st[16]), ncls=[int(v) for v in st[17:23]])
         if want_detail:
             self.stats["nref"] = nref
             self.stats["cls"] = cls
+            self.stats["lev"] = lev
+            self.stats["unc"] = unc
             if pred is not None:
                 self.stats["pred"] = pred
         return self.wnear

@@ -133,6 +133,55 @@ def get_far_order_gpu(eps, geom, cms, rads, targs, ikerorder, zk, row_ptr, col_i
     return novers, ixyzso
 
 
+def oversample_geom_gpu(geom, novers):
+    """oversample_geom and the oversampled nodes' smooth weights on the GPU through the
+    oversample_geom_ / get_qwts_ drop-ins (same result as oversample_geom); returns
+    (srcover (nptso, 12), whtsover (nptso,), ixyzso (npatches + 1,))."""
+    import ctypes as C
+    from ._lib import lib
+    npat = len(geom["norders"])
+    novers = np.ascontiguousarray(novers, np.int64)
+    ixyzso = np.ones(npat + 1, np.int64)
+    ixyzso[1:] += np.cumsum([_npols(int(n)) for n in novers])
+    nso = int(ixyzso[-1]) - 1
+    arr = lambda a, dt: np.ascontiguousarray(a, dt)  # noqa: E731
+    keep = [arr(geom["norders"], np.int64), arr(geom["ixyzs"], np.int64),
+            arr(geom.get("iptype", np.ones(npat)), np.int64), arr(geom["srccoefs"], np.float64),
+            arr(geom["srcvals"], np.float64)]
+    p = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+    i8 = lambda v: C.byref(C.c_int64(v))  # noqa: E731
+    srcover = np.zeros((nso, 12))
+    wover = np.zeros(nso)
+    L = lib()
+    L.oversample_geom_(i8(npat), p(keep[0]), p(keep[1]), p(keep[2]), i8(len(keep[4])), p(keep[3]), p(keep[4]),
+                       p(novers), p(ixyzso), i8(nso), p(srcover))
+    if L.fmm3dbie_last_error() != 0:
+        raise RuntimeError("oversample_geom failed (status %d)" % L.fmm3dbie_last_error())
+    ones = np.ones(npat, np.int64)  # iptype of the oversampled patches (kept alive for the call)
+    L.get_qwts_(i8(npat), p(novers), p(ixyzso), p(ones), i8(nso), p(srcover), p(wover))
+    if L.fmm3dbie_last_error() != 0:
+        raise RuntimeError("get_qwts failed (status %d)" % L.fmm3dbie_last_error())
+    return srcover, wover, ixyzso
+
+
+def get_qwts_gpu(geom):
+    """get_qwts on the GPU through the get_qwts_ drop-in."""
+    import ctypes as C
+    from ._lib import lib
+    npat = len(geom["norders"])
+    arr = lambda a, dt: np.ascontiguousarray(a, dt)  # noqa: E731
+    keep = [arr(geom["norders"], np.int64), arr(geom["ixyzs"], np.int64),
+            arr(geom.get("iptype", np.ones(npat)), np.int64), arr(geom["srcvals"], np.float64)]
+    p = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+    w = np.zeros(len(keep[3]))
+    L = lib()
+    L.get_qwts_(C.byref(C.c_int64(npat)), p(keep[0]), p(keep[1]), p(keep[2]), C.byref(C.c_int64(len(w))), p(keep[3]),
+                p(w))
+    if L.fmm3dbie_last_error() != 0:
+        raise RuntimeError("get_qwts failed (status %d)" % L.fmm3dbie_last_error())
+    return w
+
+
 def get_iquad_rsc(geom, row_ptr, col_ind):
     npols = np.diff(geom["ixyzs"])
     iquad = np.ones(len(col_ind) + 1, np.int64)

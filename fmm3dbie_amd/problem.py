@@ -11,7 +11,8 @@ from . import near
 def setup(geom, eps, helm, pars, far_order=True, nover_fixed=None, gpu_near=False):
     """Returns a dict with row_ptr, col_ind, iquad (1-based), rfac, rfac0, cms, rads,
     novers, ixyzso, srcover, whtsover, wts, ipatch_id, uvs_targ, nnz, nquad.
-    gpu_near: build the near list and the far-field orders on the GPU (same result)."""
+    gpu_near: build the near list, the far-field orders, the oversampled geometry and the
+    smooth weights on the GPU (same result)."""
     norder = near.norder_avg(geom)
     rfac, rfac0 = near.get_rfacs(norder)
     cms, rads = near.get_centroid_rads(geom)
@@ -35,10 +36,15 @@ def setup(geom, eps, helm, pars, far_order=True, nover_fixed=None, gpu_near=Fals
         novers, _ = far(eps, geom, cms, rads, targs, ikerorder, zk if helm else 0.0, row_ptr, col_ind, rfac)
     else:
         novers = np.asarray(geom["norders"], np.int64).copy()
-    srcover, whtsover, ixyzso = near.oversample_geom(geom, novers)
+    if gpu_near:
+        srcover, whtsover, ixyzso = near.oversample_geom_gpu(geom, novers)
+        wts = near.get_qwts_gpu(geom)
+    else:
+        srcover, whtsover, ixyzso = near.oversample_geom(geom, novers)
+        wts = near.get_qwts(geom)
     pid, uvs = near.get_patch_id_uvs(geom)
     return dict(row_ptr=row_ptr, col_ind=col_ind, iquad=iquad, rfac=rfac, rfac0=rfac0, cms=cms, rads=rads,
-                novers=novers, ixyzso=ixyzso, srcover=srcover, whtsover=whtsover, wts=near.get_qwts(geom),
+                novers=novers, ixyzso=ixyzso, srcover=srcover, whtsover=whtsover, wts=wts,
                 ipatch_id=pid, uvs_targ=uvs, nnz=len(col_ind), nquad=int(iquad[-1]) - 1)
 
 

@@ -1,41 +1,75 @@
-"""Learned refinement-depth selector for the adaptive near quadrature.
+"""Learned order selection for the adaptive near quadrature.
 
 The adaptive integrator (triaadap_main, triaintrouts.f:1269-1464) refines a (target,
-patch) pair's triangle tree until the children reproduce the parent's moments to epsp.
-Its trees are shallow: most pairs stop after refining at most the root.  An 8 -> 64 ->
-64 -> 6 MLP, trained on the GPU on the integrator's own refinement counts, predicts the
-number of refined triangles m (class min(m, 5)) from eight geometric features of the
-pair (distance and height over the patch, nearest node, tolerance, |k| rad, aspect).
-Pairs predicted m = 0 or 1 go to the cached-tree kernels, which apply the adaptive test
-to the shared depth-0..2 triangles on the matrix cores and hand failures back; the rest
-run through the adaptive kernel longest-predicted first.  The weights are those of the
-adaptive integrator either way; the prediction only decides where the work runs.
+patch) pair's triangle tree until the children reproduce the parent's moments to epsp,
+with one XG rule on every triangle (degree 16 at eps = 1e-6, 55 nodes).  Which rule is
+cheapest depends on the pair: a smooth integrand is done with a high degree and no
+refinement, a nearly singular one with a lower degree on a deeper tree; measured on the
+wavy torus, the cheapest of six degrees per pair needs 0.65 of the kernel evaluations of
+the fixed degree.  An 8 -> 64 -> 64 -> 6 MLP, trained on the GPU, predicts that rule
+class from eight geometric features of the pair (distance and height over the patch,
+nearest node, tolerance, |k| rad, aspect).  Class c runs the same adaptive integrator
+with XG degree q0 + (-4, -2, 0, +4, +8, +12)[c] (q0 the reference's degree; class 2 is
+the plain rule) and a level cap learned with it; the integrator's own test (children
+against parent, one level up) judges every triangle, and a pair whose tree needs a
+level past its class's cap is integrated again by the plain integrator.
 
-Workflow: ``X, y = collect(nq)`` (features and labels from a plain run), ``params, info =
-train(X, y)`` (GPU training, normalisation folded into the first layer), then
-``nq.run(selector=params)``.
+Labels come from the integrator itself: every adaptive pair is integrated once per class
+(forced routing, no cap) and labelled with the class of least cost, the cost being the
+pair's own work in 64-node chunks (on the cached tree, and below it where the geometry
+is computed) weighted by unit costs fitted to the measured launch times of the six
+classes; the class's level cap is the deepest tree among the training pairs labelled
+with it, and its tree cache holds no depth below that cap.
+
+Workflow: ``X, y, idx, info = collect(nq)``, ``params, tinfo = train(X, y)`` (GPU
+training, normalisation folded into the first layer), ``lmax = level_caps(y, info)``,
+then ``nq.run(selector=params, route_lmax=lmax)``.
 """
 import ctypes as C
 
 import numpy as np
 
 from ._lib import lib
+from . import basis
 from .layer import _dptr, _torch
 
 NCLASS = 6
 NFEAT = 8
 FEATURES = ("dist/rad", "height/rad", "nearest node dist/rad", "nearest node height/rad", "nearest node edge",
             "log10 epsp", "|k| rad", "rad^2/area")
+DEGREE_OFFSETS = (-4, -2, 0, 4, 8, 12)
+PLAIN_CLASS = 2
 
 
 def nparam():
     return int(lib().fmm3dbie_mlp_nparam())
 
 
-def labels_from_nref(nref):
-    """Class = number of refined triangles m = (nref - 1) / 4, capped at 5."""
-    m = (np.asarray(nref, np.int64) - 1) // 4
-    return np.minimum(np.maximum(m, 0), NCLASS - 1).astype(np.int32)
+def class_degree(q0, c):
+    """XG degree of rule class c around the plain degree q0 (as the library: not below
+    degree 8, where the children-against-parent test stops being a reliable estimate,
+    and not above 30, the largest rule in the tables)."""
+    q = q0 + DEGREE_OFFSETS[c]
+    return min(30, q) if q >= 8 else min(q0, 8)
+
+
+def xg_size(deg):
+    return len(basis.xg_rule(deg)[1])
+
+
+def plain_degree(kq):
+    """The XG degree with kq nodes (the plain rule of a generation)."""
+    for d in range(1, 31):
+        if xg_size(d) == kq:
+            return d
+    raise ValueError("no XG rule with %d nodes" % kq)
+
+
+def forced(c):
+    """Parameters whose argmax is class c for every input (all weights zero)."""
+    p = np.zeros(nparam(), np.float32)
+    p[-NCLASS + c] = 1.0
+    return p
 
 
 def init_params(seed=3):
@@ -51,16 +85,109 @@ def init_params(seed=3):
 
 
 def collect(nq, stream=None):
-    """Plain adaptive run of a NearQuadDevice; returns (features (nadap, 8) float32,
-    labels (nadap,) int32, CSR entry index of each adaptive pair)."""
+    """Features of every adaptive pair (from a plain run) and its label: the rule class
+    whose adaptive integration takes the least time.  Every pair is integrated once
+    with each class (forced routing, no level cap, every kernel in order on one stream).
+    The kernel works in chunks of 64 nodes (a wavefront's lanes; the root is
+    ceil(kq/64) chunks, a refinement test's four children ceil(4kq/64)), and a chunk
+    costs the same whether its lanes are all busy or not, so the measured time of each
+    class's launch is fitted by a cost per chunk on the cached tree (depths 0..3), a
+    cost per chunk below it (geometry evaluated) and a cost per pair, summed over the
+    pairs' own counts; the label is the class of least fitted chunk cost among the
+    classes worth their tree cache (open_classes).  Returns (X
+    (nadap, 8) float32, y (nadap,) int32, CSR entry index of each adaptive pair, info
+    with the per-class fitted cost 'cost', kernel evaluations 'evals', deepest levels
+    'lev' (6, nadap), launch times 'ms', the fitted unit costs 'unit' (ms per cached
+    chunk, per computed chunk, per pair), node counts 'kq', the plain degree 'q0', the
+    classes used 'used' and the plain tree cache's time 'ms_tree')."""
+    from scipy.optimize import nnls
+    from . import layer
     torch = _torch()
-    feat = torch.zeros((nq.nnz, NFEAT), dtype=torch.float32, device=nq.wnear.device)
-    nq.run(stream=stream, want_detail=True, features=feat)
-    torch.cuda.synchronize()
-    idx = np.nonzero(nq.stats["cls"] == 1)[0]
-    X = feat.cpu().numpy()[idx]
-    y = labels_from_nref(nq.stats["nref"][idx])
-    return X, y, idx
+    dev = nq.wnear.device
+    feat = torch.zeros((nq.nnz, NFEAT), dtype=torch.float32, device=dev)
+    side = layer.side_stream_enabled()
+    layer.set_side_stream(False)
+    try:
+        nq.run(stream=stream, want_detail=True, features=feat)
+        torch.cuda.synchronize()
+        ms_tree = nq.stats["ms_tree"]
+        idx = np.nonzero(nq.stats["cls"] == 1)[0]
+        X = feat.cpu().numpy()[idx]
+        q0 = plain_degree(nq.stats["kq"])
+        kq = np.array([xg_size(class_degree(q0, c)) for c in range(NCLASS)])
+        ncached = np.zeros((NCLASS, len(idx)))
+        ncomp = np.zeros((NCLASS, len(idx)))
+        ccached = np.zeros((NCLASS, len(idx)))
+        ccomp = np.zeros((NCLASS, len(idx)))
+        lev = np.zeros((NCLASS, len(idx)), np.int32)
+        ms = np.zeros(NCLASS)
+        for c in range(NCLASS):
+            sel = torch.as_tensor(forced(c), device=dev)
+            for rep in range(2):  # the second run is timed (tables and caches built)
+                nq.run(stream=stream, want_detail=True, selector=sel, route_lmax=[20] * NCLASS)
+            st = nq.stats
+            nref = st["nref"][idx].astype(np.float64)
+            unc = st["unc"][idx].astype(np.float64)
+            ncached[c] = kq[c] * (1 + 4 * (nref - unc))
+            ncomp[c] = kq[c] * 4 * unc
+            croot, cstep = -(-kq[c] // 64), -(-4 * kq[c] // 64)
+            ccached[c] = croot + cstep * (nref - unc)
+            ccomp[c] = cstep * unc
+            lev[c] = st["lev"][idx]
+            ms[c] = st["ms_adap"]
+    finally:
+        layer.set_side_stream(side)
+    # a cost per pair (queue, staging, weights: the same for every class) keeps the two
+    # chunk costs from absorbing it
+    A = np.stack([ccached.sum(1), ccomp.sum(1), np.full(NCLASS, float(len(idx)))], 1)
+    unit, _ = nnls(A, ms)
+    cost = unit[0] * ccached + unit[1] * ccomp
+    y, used = open_classes(cost, tree_cost(ms_tree, kq, q0))
+    return X, y, idx, dict(cost=cost, evals=ncached + ncomp, lev=lev, ms=ms, unit=unit, kq=kq, q0=q0,
+                           used=used, ms_tree=ms_tree)
+
+
+def tree_cost(ms_tree, kq, q0):
+    """Time of the tree cache of each rule class, scaled from the plain rule's (ms_tree)
+    by the node count; 0 for the plain rule, whose cache every generation builds."""
+    kq = np.asarray(kq, np.float64)
+    c = ms_tree * kq / kq[PLAIN_CLASS]
+    c[[class_degree(q0, k) == q0 for k in range(NCLASS)]] = 0.0
+    return c
+
+
+def open_classes(cost, cache):
+    """Which rule classes to use and each pair's label.  A class other than the plain one
+    costs the build of its tree cache once per generation (cache[c], same unit as cost);
+    starting from the plain class alone, the class whose pairs would save the most
+    (sum over pairs of the drop in their least cost, minus its cache) is added while that
+    saving is positive.  Returns (labels: the least-cost used class of every pair, used
+    classes as a boolean array)."""
+    ncls = cost.shape[0]
+    used = np.zeros(ncls, bool)
+    used[[c for c in range(ncls) if cache[c] == 0.0] or [PLAIN_CLASS]] = True
+    best = cost[used].min(axis=0)
+    while True:
+        gain = np.array([np.maximum(best - cost[c], 0.0).sum() - cache[c] if not used[c] else -np.inf
+                         for c in range(ncls)])
+        c = int(np.argmax(gain))
+        if not np.isfinite(gain[c]) or gain[c] <= 0.0:
+            break
+        used[c] = True
+        best = np.minimum(best, cost[c])
+    masked = np.where(used[:, None], cost, np.inf)
+    return np.argmin(masked, axis=0).astype(np.int32), used
+
+
+def level_caps(y, info):
+    """Level cap of each class: the deepest tree among the pairs labelled with it (20,
+    the integrator's own cap, for a class without pairs)."""
+    caps = np.full(NCLASS, 20, np.int32)
+    for c in range(NCLASS):
+        m = y == c
+        if m.any():
+            caps[c] = max(1, int(info["lev"][c][m].max()))
+    return caps
 
 
 def fold_normalisation(params, mean, std):

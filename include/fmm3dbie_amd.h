@@ -190,6 +190,14 @@ void get_far_order_(const double *eps, const int64_t *npatches, const int64_t *n
                     const double *srccoefs, const int64_t *ndtarg, const int64_t *ntarg, const double *targvals,
                     const int64_t *ikerorder, const double *zk, const int64_t *nnz, const int64_t *row_ptr,
                     const int64_t *col_ind, const double *rfac, int64_t *nfars, int64_t *ixyzso);
+/* src/surface_routs/surf_routs.f90:304-449  oversample_geom (computed on the GPU):
+ * srcover(12, nptso) at the RV nodes of order nfars(i) on patch i (ixyzso offsets) */
+void oversample_geom_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs, const int64_t *iptype,
+                      const int64_t *npts, const double *srccoefs, const double *srcvals, const int64_t *nfars,
+                      const int64_t *ixyzso, const int64_t *nptso, double *srcover);
+/* src/surface_routs/surf_routs.f90:1598-1651  get_qwts (computed on the GPU) */
+void get_qwts_(const int64_t *npatches, const int64_t *norders, const int64_t *ixyzs, const int64_t *iptype,
+               const int64_t *npts, const double *srcvals, double *qwts);
 /* src/common/get_fmm_thresh.f:1-90  get_fmm_thresh */
 void get_fmm_thresh_(const int64_t *nds, const int64_t *ns, const double *src, const int64_t *ndt,
                      const int64_t *nt, const double *trg, double *thresh);
@@ -217,25 +225,27 @@ int fmm3dbie_last_error(void);
 
 /* getnearquad_*_comb_dir(_eval) on device buffers.  kpar6 = (k re, k im, alpha re,
  * alpha im, beta re, beta im); helm = 0 selects the Laplace kernel (real weights).
- * stats (optional, 20 doubles): nself, nadap, nfixed, overflow, nf, kq; times in ms of
- * prep, fixed, adaptive, self, host tables, selector, tree cache, cached-tree kernels;
- * pairs accepted at depth 1 and 2 by the cached-tree kernels, pairs sent back to the
- * adaptive kernel; 3 unused.  nref_host/class_host (optional, nnz ints): refinements per
- * pair and pair class (0 self, 1 adaptive, 2 fixed).
- * mlp_params (optional, device, fmm3dbie_mlp_nparam() floats): learned refinement-depth
- * selector.  The adaptive pairs (ggq-quads.f:382-386) are routed by predicted depth to
- * the cached-tree kernels, which apply the adaptive test of triaintrouts.f:1435-1448 to
- * the depth-0..2 triangles and send failures back to the adaptive kernel; the rest run
- * longest-predicted first.  Orders 1..6; ignored for higher orders.  feat_out (optional,
- * device, nnz x 8 floats): selector features of the adaptive pairs.  pred_host
- * (optional, nnz ints): predicted class (-1 for non-adaptive pairs). */
+ * stats (optional, 24 doubles): nself, nadap, nfixed, overflow, nf, kq; times in ms of
+ * prep, fixed, adaptive, self, host tables, selector, plain tree cache, routed tree
+ * caches; rule classes used, 0, pairs sent back to the plain integrator, pairs per rule
+ * class (6), 0.  nref_host/class_host/lev_host/unc_host (optional, nnz ints): refinement
+ * tests per pair, pair class (0 self, 1 adaptive, 2 fixed), deepest level of the pair's
+ * tree, refinement tests below the cached depths 0..3 (children geometry computed).
+ * mlp_params (optional, device, fmm3dbie_mlp_nparam() floats): learned order selection.
+ * Every adaptive pair (ggq-quads.f:382-386) gets a rule class c = 0..5 from the network:
+ * the adaptive integrator of triaintrouts.f:1269-1464 with the XG rule of degree
+ * q0 + (-4, -2, 0, +4, +8, +12)[c] (q0 = the reference's degree, class 2 = the plain
+ * rule; degrees kept within 8..30) and level cap route_lmax[c] (optional host array of 6; default 20); a pair whose
+ * tree needs a level past its cap is integrated again by the plain integrator.
+ * feat_out (optional, device, nnz x 8 floats): selector features of the adaptive pairs.
+ * pred_host (optional, nnz ints): predicted class (-1 for non-adaptive pairs). */
 int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatches, const int64_t *ixyzs,
                           const double *srccoefs, const double *srcvals, int64_t ndtarg, int64_t ntarg,
                           const double *targs, const int64_t *ipatch_id, const double *uvs_targ, double eps,
                           const double *kpar6, int64_t nnz, const int64_t *row_ptr, const int64_t *col_ind,
                           const int64_t *iquad, double rfac0, double *wnear, int self_n, double self_tau_panel,
                           void *stream, double *stats, int *nref_host, int *class_host, const float *mlp_params,
-                          float *feat_out, int *pred_host);
+                          float *feat_out, int *pred_host, const int *route_lmax, int *lev_host, int *unc_host);
 
 /* findnearmem / findnear on device arrays (xyzs (3,ns), rads (ns), targets (ndt,nt));
  * row_ptr may be null for the count. */
@@ -251,10 +261,23 @@ int fmm3dbie_far_order_dev(double eps, int64_t npatches, int norder, const int64
                            const int64_t *row_ptr, const int64_t *col_ind, double rfac, int64_t *novers,
                            void *stream);
 
+/* oversample_geom and the smooth weights of the oversampled nodes (get_qwts applied to
+ * srcover, surf_routs.f90:1598-1651) on device arrays; uniform patch order norder <= 20,
+ * nfars (npatches) <= 20. */
+int fmm3dbie_oversample_geom_dev(int64_t npatches, int norder, const int64_t *ixyzs, const double *srccoefs,
+                                 const double *srcvals, const int64_t *nfars, const int64_t *ixyzso,
+                                 double *srcover, double *whtsover, void *stream);
+
 /* Near-quadrature calls keep their device work buffers (grow-only, per device) and
  * per-order tables between calls and are serialised by a lock; this frees the buffers.
  * No reference counterpart. */
 void fmm3dbie_release_workspace(void);
+/* The host lpcomp_*_addsub_ / *_eval_addsub_ drop-ins keep their last apply plan (device
+ * copies of the geometry, near weights and bookkeeping) and reuse it while every array's
+ * address, length and 257 sampled entries are unchanged, as when zgmres_guru calls them
+ * once per iteration; fmm3dbie_release_workspace() frees it.  This counts the plans they
+ * built.  No reference counterpart. */
+int64_t fmm3dbie_host_plan_builds(void);
 
 /* Near-quadrature concurrency (no reference counterpart): with enable = 1 (default) and
  * surfaces of up to 4096 patches, the tree cache, fixed-rule and self-quadrature kernels
@@ -263,10 +286,11 @@ void fmm3dbie_release_workspace(void);
  * caller's stream (per-kernel timings undisturbed). */
 void fmm3dbie_set_side_stream(int enable);
 
-/* ---- learned refinement-depth selector (no reference counterpart) --------------------
+/* ---- learned order selection (no reference counterpart) ------------------------------
  * MLP 8 -> 64 -> 64 -> 6 (ReLU, softmax), FP32, parameters packed as W1[64][8] b1[64]
- * W2[64][64] b2[64] W3[6][64] b3[6]; X is n x 8 row-major, y class ids 0..5 (class =
- * min(5, (nref - 1) / 4), the number of refined triangles of the adaptive tree). */
+ * W2[64][64] b2[64] W3[6][64] b3[6]; X is n x 8 row-major, y class ids 0..5 (the rule
+ * class of fmm3dbie_nearquad_dev whose adaptive integration of the pair evaluates the
+ * fewest kernel values). */
 int fmm3dbie_mlp_nparam(void);
 /* mean softmax cross-entropy over n samples and its gradient (device pointers) */
 int fmm3dbie_mlp_grad(const float *params, const float *X, const int *y, int64_t n, float *grad, float *loss,

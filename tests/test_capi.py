@@ -5,6 +5,7 @@ import re
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -83,3 +84,18 @@ def test_nearquad_rejects_unsupported_order():
     with pytest.raises(ValueError, match="order 10"):
         layer.NearQuadDevice(g, True, np.array([1.1, 1.1j, 1.0]), 1e-6, np.zeros(n + 1, np.int64),
                              np.zeros(0, np.int64), np.ones(1, np.int64), device="cpu")
+
+
+def test_device_entry_points_check_their_tensors():
+    """The device apply and GMRES wrappers refuse tensors the kernels cannot take (host
+    tensors, the wrong precision, the wrong length) before any GPU call."""
+    import torch
+    from fmm3dbie_amd import layer
+    with pytest.raises(ValueError):
+        layer._dens(torch.zeros(5, dtype=torch.complex128), 5, 64, "sigma")  # host tensor
+    with pytest.raises(ValueError):
+        layer._dens(np.zeros(5, complex), 5, 64, "sigma")
+    with pytest.raises(ValueError):
+        layer._dens(torch.zeros(5, dtype=torch.complex128), 5, 48, "sigma")  # precision
+    with pytest.raises(ValueError):
+        layer._weights(torch.zeros(5, dtype=torch.float64), 5)

@@ -55,9 +55,25 @@ def test_oracle_adam_first_step_is_signed_lr():
     assert np.allclose(p1, -0.01 * np.sign(g), rtol=1e-4)
 
 
-def test_labels_and_normalisation_folding():
+def test_rule_classes_levels_and_normalisation_folding():
     from fmm3dbie_amd import selector as S
-    assert list(S.labels_from_nref([1, 5, 9, 13, 17, 21, 41])) == [0, 1, 2, 3, 4, 5, 5]
+    # the rule classes around the plain degree 16 (55 nodes), 12 (33 nodes) and 22 (96)
+    assert [S.class_degree(16, c) for c in range(6)] == [12, 14, 16, 20, 24, 28]
+    assert [S.class_degree(12, c) for c in range(6)] == [8, 10, 12, 16, 20, 24]
+    assert [S.class_degree(22, c) for c in range(6)] == [18, 20, 22, 26, 30, 30]
+    assert [S.class_degree(10, c) for c in range(6)] == [8, 8, 10, 14, 18, 22]
+    assert S.plain_degree(55) == 16 and S.plain_degree(33) == 12
+    assert S.class_degree(S.plain_degree(55), S.PLAIN_CLASS) == 16
+    # forced parameters select their class for any input
+    z, _ = M.forward(S.forced(2).astype(np.float64), np.random.default_rng(0).standard_normal((5, 8)))
+    assert np.all(np.argmax(z, 1) == 2)
+    # level caps: deepest tree of the pairs labelled with each class, 20 without pairs
+    y = np.array([0, 0, 3, 3, 3])
+    lev = np.zeros((6, 5), np.int32)
+    lev[0] = [2, 4, 9, 9, 9]
+    lev[3] = [9, 9, 1, 3, 2]
+    caps = S.level_caps(y, {"lev": lev})
+    assert list(caps) == [4, 20, 20, 3, 20, 20]
     rng = np.random.default_rng(2)
     p = S.init_params(3).astype(np.float64)
     X = rng.standard_normal((20, 8)) * 3 + 1
@@ -65,3 +81,33 @@ def test_labels_and_normalisation_folding():
     zf, _ = M.forward(S.fold_normalisation(p, mean, std).astype(np.float64), X)
     zr, _ = M.forward(p, (X - mean) / std)
     assert np.allclose(zf, zr, atol=1e-4)
+
+
+def test_open_classes_pays_for_each_tree_cache():
+    """A class is used only when its pairs save more than its tree cache costs; the plain
+    class is always used; labels are the least-cost used class."""
+    from fmm3dbie_amd import selector as S
+    P = S.PLAIN_CLASS
+    n = 100
+    cost = np.full((6, n), 10.0)
+    # class 0 saves 1 on every pair (100 in total), class 5 saves 5 on 10 pairs (50, and
+    # 40 more than class 0 there)
+    cost[0] = 9.0
+    cost[5, :10] = 5.0
+    cache = np.full(6, 1e9)
+    cache[P] = 0.0
+    cache[0], cache[5] = 60.0, 30.0
+    y, used = S.open_classes(cost, cache)
+    assert used.tolist() == [True, False, P == 2, False, False, True]
+    assert np.all(y[:10] == 5) and np.all(y[10:] == 0)
+    # with a dearer cache class 5 no longer pays (its extra saving over class 0 is 40)
+    cache[5] = 45.0
+    y, used = S.open_classes(cost, cache)
+    assert not used[5] and np.all(y == 0)
+    # nothing pays: everything stays on the plain rule
+    cache[0], cache[5] = 200.0, 60.0
+    y, used = S.open_classes(cost, cache)
+    assert used.sum() == 1 and used[P] and np.all(y == P)
+    # the plain class's cache is free; classes of the plain degree too
+    c = S.tree_cost(0.3, [33, 42, 55, 79, 112, 150], 16)
+    assert c[P] == 0.0 and abs(c[5] - 0.3 * 150 / 55) < 1e-15
