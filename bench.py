@@ -6,11 +6,14 @@ Workload (default, the largest configuration): Helmholtz combined field, zk = 10
 alpha = i zk, beta = 1, on a sphere deformed radially by 1 + 0.2 sin(3 theta) cos(2 phi),
 cube-projected at 32x32 cells per face (12288 triangles, order 6, 344064 points),
 eps = 1e-6.  A step is one full near-quadrature generation over every near (target,
-patch) pair with all inputs resident in HBM.  The same JSON line carries the combined
-solve of that configuration (16 plane waves) and the solves of the smaller
-configurations (Laplace sphere, Helmholtz sphere, 64 batched plane waves, wavy torus).
+patch) pair with all inputs resident in HBM.  The same JSON line carries the applies of
+that configuration and the solves of the smaller configurations (Laplace sphere,
+Helmholtz sphere, 64 batched plane waves, wavy torus with learned order selection);
+--combined adds the combined solve of the large configuration (16 plane waves, plain
+FP64 against learned + mixed + batched), which takes about ten minutes.
 
-Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+Prints one JSON line (progress lines go to stderr).  --gpus N with N > 1 exits: this
+build runs on one GPU.
 """
 import argparse
 import json
@@ -18,9 +21,9 @@ import os
 import sys
 import time
 
-# the CPU baseline's OpenMP threads stay next to their cores (read when the runtime starts)
-os.environ.setdefault("OMP_PROC_BIND", "close")
-os.environ.setdefault("OMP_PLACES", "cores")
+# the CPUs this process may run on, read before any OpenMP runtime can bind the main
+# thread to one of them
+_AFFINITY0 = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
 
 import numpy as np
 
@@ -31,6 +34,14 @@ F_K_HELM = 40.0
 F_K_LAP = 22.0
 PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix) FLOP/s
 PEAK_HBM = 8.0e12
+
+
+_T0 = time.time()
+
+
+def _progress(msg):
+    """One line per section on stderr (the JSON line stays alone on stdout)."""
+    print("[bench %6.1f s] %s" % (time.time() - _T0, msg), file=sys.stderr, flush=True)
 
 
 def far_flops_per_pair(m):
@@ -97,6 +108,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-plane-wave batched solve")
     ap.add_argument("--no-extra", action="store_true",
                     help="only the configuration's own sections (no solves of the smaller configurations)")
+    ap.add_argument("--combined", action="store_true",
+                    help="with --config 4: the combined solve of 16 plane waves (plain FP64 against learned + "
+                         "mixed + batched; about ten minutes, so not part of the default run)")
     ap.add_argument("--combined-fp64-rhs", type=int, default=2,
                     help="right-hand sides of the 344k-point combined run solved by plain FP64 GMRES (timed, "
                          "scaled to 16) and by the converged FP64 comparator")
@@ -128,6 +142,7 @@ def main():
 
     dev = torch.device("cuda:0")
     g, helm, pars, wname, zk = workload(args.config)
+    _progress("workload %s: %d points" % (wname, len(g["srcvals"])))
     eps = 1e-6
     # set-up as the reference's solver does it (near list, quadrature offsets, far orders,
     # oversampled geometry), the near list and far orders on the GPU; on the smaller
@@ -150,6 +165,7 @@ def main():
         print(json.dumps(apply_only(g, helm, pars, P, nq, dev)))
         return
     stream = torch.cuda.current_stream()
+    _progress("set-up done; %d warm-up and %d timed generations" % (args.warmup, args.steps))
     for _ in range(args.warmup):
         nq.run()
     torch.cuda.synchronize()
@@ -234,6 +250,7 @@ def main():
             tr["fetch_scale"], tr["write_scale"])
     # GMRES solves
     if args.config == 4:
+        _progress("applies")
         out["apply"] = large_apply(g, pars, P, nq.wnear, dev)
         from fmm3dbie_amd import near
         rfac, _ = near.get_rfacs(near.norder_avg(g))
@@ -243,20 +260,26 @@ def main():
         tg = time.perf_counter() - t0
         out["findnear"] = {"gpu_s_incl_copies": tg, "same_as_setup": bool(np.array_equal(rpg, P["row_ptr"])
                                                                             and np.array_equal(cig, P["col_ind"]))}
-        if not args.no_solve:
+        if args.combined:
+            _progress("combined solve")
             out["combined_solve"] = combined_solve(g, pars, P, nq, dev, dt, args.combined_fp64_rhs)
     elif not args.no_solve and helm and args.config == 1:
         out["solve"] = sphere_solve(dev, (g, pars, P, nq.wnear))
     del nq
     if not args.no_cpu:
+        _progress("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(g, helm, pars, eps, P, args.cpu_seconds)
     if not args.no_solve and not args.no_extra:
+        _progress("laplace solve")
         out["laplace_solve"] = laplace_solve(dev)
         if args.config != 1:
+            _progress("sphere solve")
             out["sphere_solve"] = sphere_solve(dev)
         if not args.no_learned:
+            _progress("torus learned + mixed solve")
             out["learned_mixed_solve"] = learned_mixed_solve(dev)
         if not args.no_batched:
+            _progress("batched solve")
             out["batched_solve"] = batched_solve(dev, args.batched_sequential)
     print(json.dumps(out))
 
@@ -587,11 +610,15 @@ def batched_solve(dev, nseq):
             "tune_s": ttune, "apply_fp32_64rhs_ms": {"oversample": tm[0], "far": tm[1], "reduce_csr": tm[2]}}
 
 
+# CPU share of one GPU on the benchmark machines (one process, one GPU): 16 cores
+CPU_SHARE = 16
+
+
 def _host_cores():
-    """(cores usable by this process, nproc, CPU model): the affinity mask, capped by a
-    cgroup CPU quota when one is set (cpu.max)."""
-    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    usable = nproc
+    """(cores used, nproc, CPU model): the process's affinity mask at start-up, capped by a
+    cgroup CPU quota when one is set (cpu.max) and by the 16-core CPU share of one GPU."""
+    nproc = len(_AFFINITY0) if _AFFINITY0 is not None else (os.cpu_count() or 1)
+    usable = min(nproc, CPU_SHARE)
     try:
         q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
         if q != "max":
@@ -614,8 +641,15 @@ def cpu_baseline(g, helm, pars, eps, P, budget_s):
     threads bound close to their cores) on every core this process may use, over a
     bounded sample of evenly spaced target rows of the same workload; rate in near-quad
     pairs/s (pairs of the sample / time)."""
-    from oracle import oracle as O
     usable, nproc, model = _host_cores()
+    # the oracle's OpenMP runtime reads its places when it is loaded: give it the start-up
+    # affinity (another runtime in the process may have bound the main thread to one core)
+    # and keep its threads next to their cores
+    if _AFFINITY0 is not None:
+        os.sched_setaffinity(0, _AFFINITY0)
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ["OMP_PLACES"] = "cores"
+    from oracle import oracle as O
     threads = O.set_num_threads(usable)
     N = len(g["srcvals"])
     row_ptr = P["row_ptr"]
@@ -641,7 +675,8 @@ def cpu_baseline(g, helm, pars, eps, P, budget_s):
     return {"value": rate, "unit": "pairs/s", "cores": threads, "kind": "port",
             "nproc": nproc, "cpu_model": model, "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
             "sample": "%d near pairs of %d evenly spaced target rows (of %d) of the same workload, C oracle "
-                      "(same self, adaptive and fixed rules as the GPU) on %d threads" % (used, take, N, threads)}
+                      "(same self, adaptive and fixed rules as the GPU) on %d threads (%d CPUs visible, CPU share "
+                      "of one GPU %d)" % (used, take, N, threads, nproc, CPU_SHARE)}
 
 
 def combined_solve(g, pars, P, nq, dev, t_quad, nfp):
@@ -652,13 +687,14 @@ def combined_solve(g, pars, P, nq, dev, t_quad, nfp):
       GMRES one right-hand side at a time to a relative residual of RTOL, timed on nfp
       right-hand sides and scaled to 16;
     * learned + mixed + batched form: the learned selector trained on this surface's own
-      adaptive pairs (training time reported apart), the routed quadrature, the far-sum
-      launch tuned for 16 right-hand sides, and FP32-inner / FP64-refined GMRES over all
-      16 right-hand sides in lockstep to the same RTOL;
+      adaptive pairs (training time reported apart), the routed quadrature, and
+      FP32-inner / FP64-refined GMRES over all 16 right-hand sides in lockstep (one far-sum
+      launch per step for 8 right-hand sides at a time) to the same RTOL;
     * a converged FP64 solve (RTOL_REF, untimed) of the first nfp right-hand sides: the
       error of both forms against it."""
     import torch
     from fmm3dbie_amd import layer, problem, selector as S
+    layer.set_gmres_print(10)  # progress on stderr: the solves take minutes
     zk = float(np.real(pars[0]))
     dirs = problem.plane_wave_dirs(16, 4)
     rhs = torch.as_tensor(-problem.plane_waves(g["srcvals"][:, :3], dirs, zk), device=dev)
@@ -668,15 +704,21 @@ def combined_solve(g, pars, P, nq, dev, t_quad, nfp):
                            P["ixyzso"], P["srcover"], P["whtsover"])
     plan.apply(rhs[0], 64, 0)
     xs = []
-    _, tseq = _timed(lambda: [xs.append(layer.gmres(plan, rhs[r], 0.5, wts, numit=300, eps_gmres=RTOL))
-                              for r in range(nfp)])
+
+    def one(r):
+        _progress("combined: plain FP64 GMRES, right-hand side %d of %d" % (r + 1, nfp))
+        return layer.gmres(plan, rhs[r], 0.5, wts, numit=300, eps_gmres=RTOL)
+
+    _, tseq = _timed(lambda: [xs.append(one(r)) for r in range(nfp)])
     x64 = torch.stack([x[0] for x in xs]).cpu().numpy()
     it64 = [int(x[1][0]) for x in xs]
+    _progress("combined: converged FP64 comparator")
     xc, itc, _, rrc, _ = layer.gmres(plan, rhs[:nfp].contiguous(), 0.5, wts, numit=400, eps_gmres=RTOL_REF)
     xc = xc.cpu().numpy()
     plan.close()
     del w_plain
     # learned + mixed + batched
+    _progress("combined: labels and training")
     (X, y, idx, linfo), tcol = _timed(lambda: S.collect(nq))
     params, info = S.train(X, y, n_train=200000, epochs=30, seed=3, device=dev)
     lmax = S.level_caps(y, linfo)
@@ -685,7 +727,10 @@ def combined_solve(g, pars, P, nq, dev, t_quad, nfp):
     st = dict(nq.stats)
     plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], nq.wnear, P["novers"],
                            P["ixyzso"], P["srcover"], P["whtsover"])
-    tune, ttune = _timed(lambda: plan.tune(32, 16))
+    # the default far-sum shape (8 right-hand sides per group, 32768-target workgroups) is
+    # within 1% of the tuned one at this size, and tuning would time 20 shapes of 0.7-1.9 s
+    _progress("combined: mixed-precision GMRES of 16 right-hand sides")
+    ttune = 0.0
     plan.apply(rhs.to(torch.complex64), 32, 1)
     (xb, outer, inner, rr, n32, n64), tb = _timed(
         lambda: layer.gmres_ir(plan, rhs, 0.5, wts, numit=300, eps_inner=RTOL / 2, tol=RTOL))

@@ -28,6 +28,7 @@ SIGS = {
     "fmm3dbie_mlp_nparam": ([], i32),
     "fmm3dbie_release_workspace": ([], None),
     "fmm3dbie_set_side_stream": ([C.c_int], None),
+    "fmm3dbie_set_gmres_print": ([C.c_int], None),
     "findnearmem_": ([vp, vp, vp, vp, vp, vp, vp], None),
     "findnear_": ([vp, vp, vp, vp, vp, vp, vp, vp], None),
     "fmm3dbie_findnearmem_dev": ([i64, vp, vp, i64, i64, vp, vp, vp, vp], i32),

@@ -1212,6 +1212,8 @@ int64_t fmm3dbie_host_plan_builds() {
 // Concurrency of the near-quadrature kernels (see the header).
 void fmm3dbie_set_side_stream(int enable) { fmm_nearquad_set_side_stream(enable); }
 
+void fmm3dbie_set_gmres_print(int every) { gmres_set_print(every); }
+
 // ---------------------------------------------------------------- learned selector
 int fmm3dbie_mlp_nparam() { return MLP_NPARAM; }
 

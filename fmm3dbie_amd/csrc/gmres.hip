@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -19,6 +20,10 @@
 #include "gmres.hpp"
 
 namespace {
+
+// Iteration lines as the reference prints them ("iter=", it, errs(it),
+// gmres_routs.f90:303), here on stderr and every g_print-th iteration (0: none).
+std::atomic<int> g_print{0};
 
 template <typename T>
 struct C2 {
@@ -322,6 +327,12 @@ int gmres_impl(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, voi
         its[r] = it;
       }
     }
+    const int every = g_print.load();
+    if (every > 0 && it % every == 0) {
+      double mx = 0.0;
+      for (int r = 0; r < nrhs; r++) mx = std::max(mx, errs[r * numit + it - 1]);
+      fprintf(stderr, "gmres (%s, %d systems) iter= %d %.6e\n", sizeof(T) == 4 ? "fp32" : "fp64", nrhs, it, mx);
+    }
     GCHK(hipMemcpyAsync(dsc, scv.data(), sizeof(double) * 2 * nrhs, hipMemcpyHostToDevice, s));
     if (it < numit) k_scale<T><<<g2(N, nrhs), 256, 0, s>>>(N, W, N, V + it * ldv, ldvr, dsc);
     bool all = true;
@@ -366,6 +377,8 @@ int gmres_impl(ApplyPlan* P, int mode, const double* zid, const void* rhs_v, voi
 }
 
 }  // namespace
+
+void gmres_set_print(int every) { g_print.store(every > 0 ? every : 0); }
 
 int gmres_device(ApplyPlan* P, int prec, int mode, const double* zid, const void* rhs, void* soln,
                  const double* wts, int nrhs, int numit, double eps_gmres, int* niter, double* errs, double* rres,
@@ -451,6 +464,8 @@ int gmres_ir_device(ApplyPlan* P, int mode, const double* zid, const void* rhs_v
       cur[r] = e;
       worst = e > worst ? e : worst;
     }
+    if (g_print.load() > 0)
+      fprintf(stderr, "gmres refinement %d: %d inner iterations, fp64 residual %.3e\n", outer, mx, worst);
     if (worst <= tol) break;
   }
   if (outer > max_outer) outer = max_outer;

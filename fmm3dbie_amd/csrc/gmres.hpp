@@ -17,3 +17,5 @@ int gmres_device(ApplyPlan* P, int prec, int mode, const double* zid, const void
 int gmres_ir_device(ApplyPlan* P, int mode, const double* zid, const void* rhs, void* soln, const double* wts,
                     int nrhs, int numit, double eps_inner, double tol, int max_outer, int* outer, int* inner_total,
                     double* rres, hipStream_t s, GmresStats* gs);
+// iteration lines on stderr every `every` iterations (0: none)
+void gmres_set_print(int every);

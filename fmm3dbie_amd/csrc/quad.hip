@@ -585,8 +585,9 @@ __device__ __forceinline__ Mom column_sum_rows(const double* pb, const double2* 
 // (64 x 16 bytes per load-to-LDS instruction, no registers) while the lanes read the
 // nodes' geometry from the tree cache and evaluate the kernel.  Below the cached depths
 // each lane writes its node's basis row and sums the geometry from it and the patch
-// coefficients sc in LDS.
-template <int N, bool HELM>
+// coefficients sc in LDS.  CACHED selects the path at compile time (the caller tests the
+// step's tree id), so each path is scheduled without the other's registers.
+template <int N, bool HELM, bool CACHED>
 __device__ __forceinline__ void step_moments(double* pb, double2* fb, const double* sc, bool root, int npar,
                                              int tid, double x0, double y0, double dx, double dy, int kq,
                                              const double* __restrict__ xg, const double* __restrict__ cpatch,
@@ -598,7 +599,7 @@ __device__ __forceinline__ void step_moments(double* pb, double2* fb, const doub
   const int ntri = root ? 1 : 4;
   const int total = ntri * kq;
   // cached: the root itself, or children of a cached triangle of depth <= 2
-  const bool cached = root ? true : (tid >= 0 && tid < npar);
+  constexpr bool cached = CACHED;
   const int t0 = root ? 0 : cached_child(tid, 0);  // first cached tree id of the step
   const double hx = 0.5 * dx, hy = 0.5 * dy, da = hx * hx;
 #pragma unroll
@@ -733,7 +734,7 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
     const int npar = R.npar[cls];
     const double* cpatch = R.cache[cls] + (size_t)ip * CACHE_STRIDE * cache_tris(npar) * kq;
     Mom mc[4];
-    step_moments<N, HELM>(pb, fb, lsc, true, npar, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
+    step_moments<N, HELM, true>(pb, fb, lsc, true, npar, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
     if (lane < 6) stack[lane] = lane < 2 ? 0.0 : (lane < 4 ? 1.0 : 0.0);  // level 0, tree id 0
     if (lane < NP) {
       stack[6 + lane] = mc[0].re;
@@ -750,7 +751,10 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
       const int level = __builtin_amdgcn_readfirstlane((int)e[4]);
       const int tid = __builtin_amdgcn_readfirstlane((int)e[5]);  // cached tree id, -1 below depth 2
       const Mom mpar = lane < NP ? Mom{e[6 + lane], e[6 + NP + lane]} : Mom{0.0, 0.0};
-      step_moments<N, HELM>(pb, fb, lsc, false, npar, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
+      if (tid >= 0 && tid < npar)
+        step_moments<N, HELM, true>(pb, fb, lsc, false, npar, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
+      else
+        step_moments<N, HELM, false>(pb, fb, lsc, false, npar, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
       nref++;
       nunc += (tid < 0 || tid >= npar) ? 1 : 0;  // children below the cached tree
       Mom ssum = mc[0];

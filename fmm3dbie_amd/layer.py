@@ -285,6 +285,13 @@ def set_side_stream(enable=True):
 _SIDE = True
 
 
+def set_gmres_print(every=1):
+    """GMRES progress on stderr: the iteration and residual estimate every `every`
+    iterations (as the reference's zgmres_guru prints each iteration), and a line per
+    refinement of gmres_ir; 0 turns it off (the default)."""
+    lib().fmm3dbie_set_gmres_print(int(every))
+
+
 def side_stream_enabled():
     """The current side-stream setting (set_side_stream; on by default)."""
     return _SIDE

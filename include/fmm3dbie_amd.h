@@ -286,6 +286,12 @@ int64_t fmm3dbie_host_plan_builds(void);
  * caller's stream (per-kernel timings undisturbed). */
 void fmm3dbie_set_side_stream(int enable);
 
+/* GMRES progress (the reference's zgmres_guru prints "iter=", it, errs(it) every
+ * iteration, gmres_routs.f90:303): every > 0 prints the iteration number and the largest
+ * relative residual estimate over the systems to stderr every `every` iterations, and a
+ * line per refinement of fmm3dbie_gmres_ir_dev; 0 (default) prints nothing. */
+void fmm3dbie_set_gmres_print(int every);
+
 /* ---- learned order selection (no reference counterpart) ------------------------------
  * MLP 8 -> 64 -> 64 -> 6 (ReLU, softmax), FP32, parameters packed as W1[64][8] b1[64]
  * W2[64][64] b2[64] W3[6][64] b3[6]; X is n x 8 row-major, y class ids 0..5 (the rule

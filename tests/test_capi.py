@@ -99,3 +99,20 @@ def test_device_entry_points_check_their_tensors():
         layer._dens(torch.zeros(5, dtype=torch.complex128), 5, 48, "sigma")  # precision
     with pytest.raises(ValueError):
         layer._weights(torch.zeros(5, dtype=torch.float64), 5)
+
+
+def test_setup_passes_the_complex_wavenumber_to_the_far_order():
+    """problem.setup hands zpars(1) to get_far_order as the complex number the reference
+    passes (helm_comb_dir.f:596, 2010): the orders equal the oracle's for zk = 1.1 + 3i,
+    and differ from those of the real part alone (the decaying kernel needs less
+    oversampling on some patches)."""
+    from fmm3dbie_amd import geometry as G, near, problem
+    from oracle import oracle as O
+    g = G.get_sphere_npat(1.0, 2, (0, 0, 0), 3)
+    zk = 1.1 + 3j
+    P = problem.setup(g, 1e-6, True, np.array([zk, 1j * zk, 1.0]))
+    targs = g["srcvals"][:, :3]
+    ref, _ = O.far_order(1e-6, g, P["cms"], P["rads"], targs, 0, zk, P["row_ptr"], P["col_ind"], P["rfac"])
+    assert np.array_equal(P["novers"], ref)
+    real, _ = O.far_order(1e-6, g, P["cms"], P["rads"], targs, 0, 1.1, P["row_ptr"], P["col_ind"], P["rfac"])
+    assert np.all(P["novers"] <= real) and not np.array_equal(P["novers"], real)
