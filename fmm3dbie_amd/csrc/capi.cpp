@@ -106,7 +106,7 @@ int nearquad_host(bool helm, int64_t npatches, const int64_t* norders, const int
   if (rc == 0) {
     (void)hipMemsetAsync(dw, 0, wn * sizeof(double), s);
     A.wnear = dw;
-    A.self_n = 10;
+    A.self_n = 8;
     A.self_tau_panel = 1.0;
     rc = fmm_nearquad_device(A, nullptr, s);
   }
@@ -633,7 +633,7 @@ int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatc
   A.beta[0] = kpar6[4]; A.beta[1] = kpar6[5];
   A.nnz = nnz; A.row_ptr = row_ptr; A.col_ind = col_ind; A.iquad = iquad;
   A.rfac0 = rfac0; A.wnear = wnear;
-  A.self_n = self_n > 0 ? self_n : 10;
+  A.self_n = self_n > 0 ? self_n : 8;
   A.self_tau_panel = self_tau_panel > 0 ? self_tau_panel : 1.0;
   A.want_nref = nref_host != nullptr;
   A.nref_host = nref_host;
@@ -652,7 +652,7 @@ int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatc
                           st.ms_adap,        st.ms_self,         st.ms_tables,       st.ms_select,
                           st.ms_tree,        st.ms_shallow,      (double)st.nroute,  0,
                           (double)st.nfallback, (double)st.ncls[0], (double)st.ncls[1], (double)st.ncls[2],
-                          (double)st.ncls[3], (double)st.ncls[4], (double)st.ncls[5], 0};
+                          (double)st.ncls[3], (double)st.ncls[4], (double)st.ncls[5], (double)st.cache_tris};
     std::memcpy(stats, v, sizeof(v));
   }
   g_last_error = rc;

@@ -478,28 +478,34 @@ __device__ __forceinline__ void store_weights(double* buf, Mom mom, const double
 }
 
 // ------------------------------------------------------------------ 4. adaptive
-// Depths 0..3 of the refinement tree (1 + 4 + 16 + 64 triangles: root 0, children of
-// the root 1..4, children of t in 1..4 at 5 + 4(t-1) + c, children of t in 5..20 at
-// 21 + 4(t-5) + c) are the same for every target of a patch, as in the reference's
-// per-patch tree (triaintrouts.f:939-1033).  Their node geometry is computed once per
-// patch into HBM (one 64-byte record per node: x, y, z, nx, ny, nz, |J|*da*w, pad, so a
-// node is read from one base address) and the basis values at those nodes once per order.
-constexpr int NCACHE_TRI = 85;
+// The first depths of the refinement tree (1 + 4 + 16 + 64 + 256 triangles: root 0,
+// children of the root 1..4, children of t in 1..4 at 5 + 4(t-1) + c, of t in 5..20 at
+// 21 + 4(t-5) + c, of t in 21..84 at 85 + 4(t-21) + c) are the same for every target of
+// a patch, as in the reference's per-patch tree (triaintrouts.f:939-1033).  Their node
+// geometry is computed once per patch into HBM (one 64-byte record per node: x, y, z, nx,
+// ny, nz, |J|*da*w, pad, so a node is read from one base address) for depths 0..3 or
+// 0..4, and the basis values at those nodes once per order.
+constexpr int NCACHE_TRI = 341;
 constexpr int CACHE_STRIDE = 8;
-constexpr int NCACHE_PARENT = 21;  // triangles whose children are cached (depths 0..2)
+constexpr int NCACHE_PARENT = 85;  // triangles whose children can be cached (depths 0..3)
 
 // Rows of the cached-tree basis table are padded to an even length, so every row starts on
 // a 16-byte boundary and is read with 16-byte loads.
 __host__ __device__ constexpr int ptab_w(int np) { return (np + 1) & ~1; }
 
 __host__ __device__ __forceinline__ int cached_child(int t, int c) {
-  return t < 0 ? -1 : (t == 0 ? 1 + c : (t <= 4 ? 5 + 4 * (t - 1) + c : (t <= 20 ? 21 + 4 * (t - 5) + c : -1)));
+  return t < 0 ? -1
+               : (t == 0 ? 1 + c
+                         : (t <= 4 ? 5 + 4 * (t - 1) + c
+                                   : (t <= 20 ? 21 + 4 * (t - 5) + c : (t <= 84 ? 85 + 4 * (t - 21) + c : -1))));
 }
 
 // The cache of a rule may hold fewer depths (ntri = 1 + 4 + ... triangles of the first
 // depths, in tree order): a rule class whose pairs stop at depth 1 or 2 needs no more.
 __host__ __device__ constexpr int cache_tris(int npar) { return 4 * npar + 1; }
-__host__ __device__ constexpr int cache_parents(int depth) { return depth <= 1 ? 1 : (depth == 2 ? 5 : NCACHE_PARENT); }
+__host__ __device__ constexpr int cache_parents(int depth) {
+  return depth <= 1 ? 1 : (depth == 2 ? 5 : (depth == 3 ? 21 : NCACHE_PARENT));
+}
 
 template <int N>
 __global__ void k_tree_cache(int64_t npatches, int kq, int ntri, const double* __restrict__ tuv,
@@ -1242,7 +1248,7 @@ void cached_tree_tables(const std::vector<double>& xq, const std::vector<double>
   for (int t = 0; t < NCACHE_PARENT; t++)
     for (int c = 0; c < 4; c++) tris[cached_child(t, c)] = kids(tris[t], c);
   for (int t = 0; t < NCACHE_TRI; t++) {
-    const double da = t == 0 ? 1.0 : (t <= 4 ? 0.25 : (t <= 20 ? 0.0625 : 0.015625));
+    const double da = t == 0 ? 1.0 : (t <= 4 ? 0.25 : (t <= 20 ? 0.0625 : (t <= 84 ? 0.015625 : 0.00390625)));
     for (int q = 0; q < kq; q++) {
       const int n = t * kq + q;
       tuv[2 * n] = tris[t].x0 + tris[t].dx * xq[2 * q];
@@ -1431,7 +1437,16 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     }
     s2 = it->second;
   }
-  const int ncn = NCACHE_TRI * kq;
+  // Depth of the plain rule's tree cache: 0..4 when the surface has many near pairs per
+  // patch, else 0..3.  Depth 4 costs one more pass over 256 triangles per patch (measured
+  // 4.5e-4 ms per patch) and saves each refinement test of a depth-3 triangle its
+  // geometry (about 8e-6 ms); on the benchmark surfaces a third of the near pairs are
+  // adaptive and 8-22% of those reach depth 4, which pays from about 300 near pairs per
+  // patch (sphere 12x12 order 4: 235, depth 3; torus 48x16 order 6: 510 and deformed
+  // sphere 32x32 order 6: 436, depth 4, 6-12% less adaptive time including the cache).
+  const int npar_plain = cache_parents(A.nnz >= 300 * npat ? 4 : 3);
+  const int ntri_plain = cache_tris(npar_plain);
+  const int ncn = ntri_plain * kq;
   double *d_tuv = TB.tuv, *d_twq = TB.twq, *d_ptab = TB.ptab;
   const bool early_tree = s2 != s && A.ipatch_id != nullptr && npat > 0;
   double* d_cache = nullptr;
@@ -1449,7 +1464,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     HCHK(hipEventRecord(ev[14], s));
     HCHK(hipStreamWaitEvent(s2, ev[14], 0));
     HCHK(hipEventRecord(ev[9], s2));
-    k_tree_cache<N><<<npat, 256, 0, s2>>>(npat, kq, NCACHE_TRI, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs,
+    k_tree_cache<N><<<npat, 256, 0, s2>>>(npat, kq, ntri_plain, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs,
                                           d_cache);
     HCHK(hipGetLastError());
     HCHK(hipEventRecord(ev[8], s2));
@@ -1575,7 +1590,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     R.seg[1] = nadap;
     R.kq[0] = kq;
     R.lmax[0] = NLMAX;
-    R.npar[0] = NCACHE_PARENT;
+    R.npar[0] = npar_plain;
     R.xg[0] = d_xg;
     R.ptab[0] = d_ptab;
   }
@@ -1585,7 +1600,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
       HCHK(hipStreamWaitEvent(s, ev[8], 0));
     } else {
       HCHK(hipEventRecord(ev[9], s));
-      k_tree_cache<N><<<npat, 256, 0, s>>>(npat, kq, NCACHE_TRI, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs,
+      k_tree_cache<N><<<npat, 256, 0, s>>>(npat, kq, ntri_plain, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs,
                                            d_cache);
       HCHK(hipEventRecord(ev[8], s));
     }
@@ -1605,13 +1620,13 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
         if (seg[c + 1] == seg[c]) continue;
         nroute++;
         if (q == nqorder) {
-          R.npar[c] = NCACHE_PARENT;
+          R.npar[c] = npar_plain;
           R.cache[c] = d_cache;
           continue;
         }
         // the class's pairs evaluate no triangle below depth lmax (deeper ones go back to
         // the plain integrator), so its cache holds depths 0 .. min(3, lmax)
-        R.npar[c] = cache_parents(R.lmax[c]);
+        R.npar[c] = cache_parents(R.lmax[c] < 3 ? R.lmax[c] : 3);
         const int ntri = cache_tris(R.npar[c]);
         double* cc = wsalloc<double>(W_RCACHE0 + c, (size_t)npat * CACHE_STRIDE * ntri * RT->kq);
         if (!cc) return -2;
@@ -1636,7 +1651,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
       P0.seg[1] = 0x7fffffff;
       P0.kq[0] = kq;
       P0.lmax[0] = NLMAX;
-      P0.npar[0] = NCACHE_PARENT;
+      P0.npar[0] = npar_plain;
       P0.xg[0] = d_xg;
       P0.ptab[0] = d_ptab;
       P0.cache[0] = d_cache;
@@ -1683,6 +1698,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     st->ms_adap = elapsed(7, 3);
     st->ms_self = elapsed(12, 13);
     st->nroute = nroute;
+    st->cache_tris = ntri_plain;
     st->nfallback = counts8[4];
     for (int c = 0; c < MAX_RULES; c++) st->ncls[c] = learned ? seg[c + 1] - seg[c] : (c == 0 ? nadap : 0);
     st->nref_total = 0;

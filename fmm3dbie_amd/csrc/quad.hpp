@@ -48,6 +48,7 @@ struct NearQuadStats {
   // tree cache, the routed classes' tree caches; ms_adap is the adaptive kernel launches.
   int nroute, nfallback, ncls[6];
   double ms_tables, ms_select, ms_tree, ms_shallow;
+  int cache_tris;  // triangles of the plain rule's tree cache (85: depths 0..3, 341: 0..4)
 };
 
 // Calls are serialised; device work buffers and per-order tables are kept between calls

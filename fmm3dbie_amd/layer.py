@@ -29,6 +29,11 @@ from ._lib import lib
 from . import near
 
 
+# Gauss-Legendre points per direction of the self rule's panels (radial and sinh-graded
+# edge parameter): 8 x 8 integrates the self-patch weights of the benchmark surfaces to
+# the same 1e-8 (relative to the largest weight of the target) as 10 x 10 or 20 x 20.
+SELF_N = 8
+
 def _p(a):
     return C.c_void_p(a.ctypes.data)
 
@@ -372,7 +377,7 @@ class NearQuadDevice:
             _dptr(self.d_srccoefs), _dptr(self.d_srcvals), self.ndtarg, self.ntarg, _dptr(self.d_targs),
             _dptr(self.d_pid) if self.d_pid is not None else None,
             _dptr(self.d_uvs) if self.d_uvs is not None else None, self.eps, _p(self.kp), self.nnz,
-            _dptr(self.d_row), _dptr(self.d_col), _dptr(self.d_iquad), self.rfac0, _dptr(self.wnear), 10, 1.0,
+            _dptr(self.d_row), _dptr(self.d_col), _dptr(self.d_iquad), self.rfac0, _dptr(self.wnear), SELF_N, 1.0,
             C.c_void_p(s.cuda_stream), _p(st), _p(nref) if want_detail else None, _p(cls) if want_detail else None,
             _dptr(selector) if selector is not None else None, _dptr(features) if features is not None else None,
             _p(pred) if pred is not None else None, _p(lm) if lm is not None else None,
@@ -382,7 +387,8 @@ class NearQuadDevice:
         self.stats = dict(nself=int(st[0]), nadap=int(st[1]), nfixed=int(st[2]), overflow=int(st[3]), nf=int(st[4]),
                           kq=int(st[5]), ms_prep=st[6], ms_fixed=st[7], ms_adap=st[8], ms_self=st[9],
                           ms_tables=st[10], ms_select=st[11], ms_tree=st[12], ms_tree_routed=st[13],
-                          nroute=int(st[14]), nfallback=int(st[16]), ncls=[int(v) for v in st[17:23]])
+                          nroute=int(st[14]), nfallback=int(st[16]), ncls=[int(v) for v in st[17:23]],
+                          cache_tris=int(st[23]))
         if want_detail:
             self.stats["nref"] = nref
             self.stats["cls"] = cls

@@ -99,7 +99,7 @@ def collect(nq, stream=None):
     with the per-class fitted cost 'cost', kernel evaluations 'evals', deepest levels
     'lev' (6, nadap), launch times 'ms', the fitted unit costs 'unit' (ms per cached
     chunk, per computed chunk, per pair), node counts 'kq', the plain degree 'q0', the
-    classes used 'used' and the plain tree cache's time 'ms_tree')."""
+    classes used 'used' and the plain rule's tree-cache time for depths 0..3 'ms_tree')."""
     from scipy.optimize import nnls
     from . import layer
     torch = _torch()
@@ -110,7 +110,7 @@ def collect(nq, stream=None):
     try:
         nq.run(stream=stream, want_detail=True, features=feat)
         torch.cuda.synchronize()
-        ms_tree = nq.stats["ms_tree"]
+        ms_tree = nq.stats["ms_tree"] * 85.0 / max(nq.stats["cache_tris"], 1)  # per depth-0..3 cache
         idx = np.nonzero(nq.stats["cls"] == 1)[0]
         X = feat.cpu().numpy()[idx]
         q0 = plain_degree(nq.stats["kq"])
@@ -148,8 +148,9 @@ def collect(nq, stream=None):
 
 
 def tree_cost(ms_tree, kq, q0):
-    """Time of the tree cache of each rule class, scaled from the plain rule's (ms_tree)
-    by the node count; 0 for the plain rule, whose cache every generation builds."""
+    """Time of the tree cache (depths 0..3) of each rule class, scaled from the plain
+    rule's (ms_tree, for depths 0..3) by the node count; 0 for the plain rule, whose cache
+    every generation builds."""
     kq = np.asarray(kq, np.float64)
     c = ms_tree * kq / kq[PLAIN_CLASS]
     c[[class_degree(q0, k) == q0 for k in range(NCLASS)]] = 0.0

@@ -225,10 +225,13 @@ int fmm3dbie_last_error(void);
 
 /* getnearquad_*_comb_dir(_eval) on device buffers.  kpar6 = (k re, k im, alpha re,
  * alpha im, beta re, beta im); helm = 0 selects the Laplace kernel (real weights).
+ * self_n, self_tau_panel: Gauss-Legendre points per direction of the self rule's panels
+ * and their largest sinh-parameter length (<= 0: 8 and 1.0).
  * stats (optional, 24 doubles): nself, nadap, nfixed, overflow, nf, kq; times in ms of
  * prep, fixed, adaptive, self, host tables, selector, plain tree cache, routed tree
  * caches; rule classes used, 0, pairs sent back to the plain integrator, pairs per rule
- * class (6), 0.  nref_host/class_host/lev_host/unc_host (optional, nnz ints): refinement
+ * class (6), triangles of the plain rule's tree cache (85: depths 0..3; 341: depths 0..4,
+ * used from 300 near pairs per patch).  nref_host/class_host/lev_host/unc_host (optional, nnz ints): refinement
  * tests per pair, pair class (0 self, 1 adaptive, 2 fixed), deepest level of the pair's
  * tree, refinement tests below the cached depths 0..3 (children geometry computed).
  * mlp_params (optional, device, fmm3dbie_mlp_nparam() floats): learned order selection.
