@@ -490,8 +490,13 @@ constexpr int CACHE_STRIDE = 8;
 constexpr int NCACHE_PARENT = 85;  // triangles whose children can be cached (depths 0..3)
 
 // Rows of the cached-tree basis table are padded to an even length, so every row starts on
-// a 16-byte boundary and is read with 16-byte loads.
-__host__ __device__ constexpr int ptab_w(int np) { return (np + 1) & ~1; }
+// a 16-byte boundary and is read with 16-byte loads, and to a length that is not a
+// multiple of 8 doubles: the adaptive kernel's lanes write their rows of the same layout
+// in LDS, and a row stride of a multiple of 64 bytes would put the lanes' writes into
+// few banks (order 4, 15 basis functions: rows of 18 instead of 16 doubles, 5% faster).
+__host__ __device__ constexpr int ptab_w(int np) {
+  return ((np + 1) & ~1) % 8 == 0 ? ((np + 1) & ~1) + 2 : ((np + 1) & ~1);
+}
 
 __host__ __device__ __forceinline__ int cached_child(int t, int c) {
   return t < 0 ? -1
@@ -639,7 +644,7 @@ __device__ __forceinline__ void step_moments(double* pb, double2* fb, const doub
         const double cx = (c & 1) ? x0 + hx : x0, cy = (c & 2) ? y0 + hy : y0;
         const double cdx = c == 3 ? -hx : hx, cdy = c == 3 ? -hy : hy;
         const double u = cx + cdx * xg[3 * q], v = cy + cdy * xg[3 * q + 1];
-        koorn_geom_row<N, true>(u, v, sc, g, pb + lane * PW);
+        koorn_geom_row<N>(u, v, sc, g, pb + lane * PW);
         w = geom_finish(g) * da * xg[3 * q + 2];
       }
       double vr, vi;
@@ -692,8 +697,6 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
   constexpr int ES = 6 + 2 * NP;  // stack entry: x0,y0,dx,dy,level,tree id, moments
   __shared__ __align__(16) double pb[WAVE * ptab_w(NP)];
   __shared__ __align__(16) double2 fb[WAVE];
-  __shared__ double lsc[9 * NP];
-  __shared__ int sh_idx;
   const int lane = threadIdx.x;
   double* stack = stack_all + (size_t)blockIdx.x * STACK_CAP * ES;
   const int nl = nlist_dev ? *nlist_dev : nlist;
@@ -704,8 +707,10 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
   const int home = blockIdx.x & 7;
   int tried = 0;  // segments found empty (lane 0)
   for (;;) {
+    // lane 0 claims the next pair; the whole wavefront is active here, so lane 0 is the
+    // first active lane and readfirstlane broadcasts its claim
+    int got = -1;
     if (lane == 0) {
-      int got = -1;
       while (tried < 8) {
         const int sg = (home + tried) & 7;
         const int a = (int)((int64_t)nl * sg / 8), b = (int)((int64_t)nl * (sg + 1) / 8);
@@ -716,11 +721,8 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
         }
         tried++;
       }
-      sh_idx = got;
     }
-    __syncthreads();
-    const int idx = __builtin_amdgcn_readfirstlane(sh_idx);
-    __syncthreads();
+    const int idx = __builtin_amdgcn_readfirstlane(got);
     if (idx < 0) break;
     int cls = 0;
     while (cls + 1 < R.nrules && idx >= R.seg[cls + 1]) cls++;
@@ -730,10 +732,9 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
     const double* ptab = R.ptab[cls];
     const int2 jt = list[idx];
     const int64_t ip = col_ind[jt.x] - 1;
-    {
-      const double* gsc = srccoefs + 9 * (ixyzs[ip] - 1);
-      for (int i = lane; i < 9 * NP; i += WAVE) lsc[i] = gsc[i];
-    }
+    // patch coefficients for the geometry below the cached tree, read with wave-uniform
+    // (scalar) loads: in LDS they would cost the LDS of a wavefront per CU at order 6
+    const double* lsc = srccoefs + 9 * (ixyzs[ip] - 1);
     const double* tp = targs + ndtarg * (int64_t)jt.y;
     const double tg[3] = {tp[0], tp[1], tp[2]};
     const double eps = epsp_patch[ip];
