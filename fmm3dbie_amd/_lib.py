@@ -21,6 +21,7 @@ SIGS = {
     "fmm3dbie_rv_umat": ([i32, vp], i32),
     "fmm3dbie_koorn_pols": ([f64, f64, i32, vp], None),
     "fmm3dbie_xg_rule": ([i32, vp, vp], i32),
+    "fmm3dbie_sincos_red": ([i64, vp, vp, vp], i32),
     "get_rfacs_": ([vp, vp, vp, vp], None),
     "fmm3dbie_last_error": ([], i32),
     "fmm3dbie_nearquad_dev": ([i32, i32, i32, i64, vp, vp, vp, i64, i64, vp, vp, vp, f64, vp, i64, vp, vp, vp,

@@ -115,7 +115,7 @@ __device__ __forceinline__ void pair_kernel(T dx, T dy, T dz, T nx, T ny, T nz, 
     c = __builtin_amdgcn_cosf(rev);
   } else {
     (void)krev;
-    sincos(kr * r, &s, &c);
+    sincos_red(kr * r, s, c);
   }
   T mag = rinv;
   if constexpr (GEN) {
@@ -146,7 +146,7 @@ __device__ __forceinline__ void pair_factors(T r2, T rinv, T d, T kr, T krev, T 
     c = __builtin_amdgcn_cosf(rev);
   } else {
     (void)krev;
-    sincos(kr * r, &s, &c);
+    sincos_red(kr * r, s, c);
   }
   if constexpr (GEN) {
     T mag;
@@ -598,6 +598,12 @@ int apply_plan_build(ApplyPlan* P, const ApplyPlanInput& in, hipStream_t s) {
   const double diag = std::sqrt((mx[0] - mn[0]) * (mx[0] - mn[0]) + (mx[1] - mn[1]) * (mx[1] - mn[1]) +
                                 (mx[2] - mn[2]) * (mx[2] - mn[2]));
   P->phase_small = !in.helm || std::fabs(in.zk[0]) * diag * 0.15915494309189533577 < 200.0;
+  // the FP64 kernel evaluations take sin/cos of k r for |k r| <= SINCOS_RED_MAX
+  if (in.helm && std::fabs(in.zk[0]) * diag > SINCOS_RED_MAX) {
+    fprintf(stderr, "fmm3dbie_amd: |Re k| x surface diameter = %g exceeds %g\n", std::fabs(in.zk[0]) * diag,
+            SINCOS_RED_MAX);
+    return -6;
+  }
 
   P->ixyzs = own<int64_t>(P, npat + 1);
   P->ixyzso = own<int64_t>(P, npat + 1);

@@ -18,6 +18,7 @@
 #include "gmres.hpp"
 #include "host_basis.hpp"
 #include "far.hpp"
+#include "trig.hpp"
 #include "learned.hpp"
 #include "near.hpp"
 #include "quad.hpp"
@@ -1213,6 +1214,14 @@ int64_t fmm3dbie_host_plan_builds() {
 void fmm3dbie_set_side_stream(int enable) { fmm_nearquad_set_side_stream(enable); }
 
 void fmm3dbie_set_gmres_print(int every) { gmres_set_print(every); }
+
+int fmm3dbie_sincos_red(int64_t n, const double* x, double* s, double* c) {
+  for (int64_t i = 0; i < n; i++) {
+    if (!(std::fabs(x[i]) <= SINCOS_RED_MAX)) return -1;
+    sincos_red(x[i], s[i], c[i]);
+  }
+  return 0;
+}
 
 // ---------------------------------------------------------------- learned selector
 int fmm3dbie_mlp_nparam() { return MLP_NPARAM; }

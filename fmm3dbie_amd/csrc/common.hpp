@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "trig.hpp"
+
 #define FMM_MAX_ORDER 10
 #define FMM_MAX_NP 66  // (10+1)*(10+2)/2
 #define OVER4PI 0.07957747154594767
@@ -89,8 +91,9 @@ __device__ __forceinline__ void koorn_eval(double u, double v, double* P) {
 
 
 // ---- FP64 elementary functions for the kernel evaluations ------------------------
-// sin/cos come from the device math library (sincos, inlined): measured faster here than
-// a hand-written Cody-Waite + Taylor version whose out-of-range fallback was a call.
+// sin/cos: sincos_red (trig.hpp), valid for |k r| <= 1.6e6, with no large-argument path:
+// the device library's sincos evaluates a large-argument reduction beside the small one
+// and selects, which made it most of a node evaluation's instructions.
 // 1/sqrt(x) for x > 0: hardware estimate refined by two Newton steps.
 __device__ __forceinline__ double rsqrt_nr(double x) {
   double y = __builtin_amdgcn_rsq(x);
@@ -117,7 +120,7 @@ __device__ __forceinline__ void helm_comb(double dx, double dy, double dz, doubl
   const double r = r2 * rinv;
   const double d = dx * nx + dy * ny + dz * nz;
   double s, c;
-  sincos(kp.zkr * r, &s, &c);
+  sincos_red(kp.zkr * r, s, c);
   double er = c, ei = s;
   if (kp.zki != 0.0) {
     const double dec = exp(-kp.zki * r);

@@ -215,6 +215,9 @@ int fmm3dbie_rv_umat(int norder, double *umat);
 void fmm3dbie_koorn_pols(double u, double v, int nmax, double *pols);
 /* src/tria_routs/triasymq.f:89-158  triasymq on the simplex: returns the node count */
 int fmm3dbie_xg_rule(int deg, double *uvs, double *wts);
+/* No reference counterpart: the sine and cosine the Helmholtz kernel evaluations use (the
+ * device function evaluated on the host), for |x| <= 1.6e6; returns -1 beyond that. */
+int fmm3dbie_sincos_red(int64_t n, const double *x, double *s, double *c);
 
 /* ---- device-pointer interface (inputs resident in HBM, caller's stream) -------------- */
 /* No single reference counterpart: the same computations as the entries above with
