@@ -292,6 +292,124 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
   }
 }
 
+// Far sum of 16 right-hand sides on the matrix cores (FP32, near patches masked): with
+// G = e^{ikr}/r and H = t (1 - ikr) e^{ikr}/r for a pair (target, source), the group's
+// potentials are the complex products pot = G q + H m of the target x source kernel
+// matrices with the source x 16 strength matrices.  A wavefront takes 16 targets; in a
+// step of 4 sources lane l evaluates the pair (target l % 16, source l / 16) once, and
+// v_mfma_f32_16x16x4f32 multiplies the 16 x 4 blocks Re G, Im G, Re H, Im H by the
+// 4 x 16 strength blocks (lane l supplies source l / 16, right-hand side l % 16): eight
+// products per step give the real and imaginary parts of 16 targets x 16 right-hand
+// sides.  Accumulator element r of lane l is target 4 (l / 16) + r, right-hand side l % 16.
+// The workgroup stages FT sources at a time in LDS: geometry (x, y, z, nx, ny, nz) and
+// the (q, m) of the 16 right-hand sides of each source.
+constexpr int FT = 128;        // sources per LDS tile
+constexpr int FW = 4;          // wavefronts per workgroup (16 targets each)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <bool GEN>
+__global__ void __launch_bounds__(64 * FW) k_farsum_mfma(int64_t ntarg, const float* __restrict__ tx,
+                                                        const float* __restrict__ ty, const float* __restrict__ tz,
+                                                        int64_t nsrc, const float* __restrict__ sx,
+                                                        const float* __restrict__ sy, const float* __restrict__ sz,
+                                                        const float* __restrict__ snx, const float* __restrict__ sny,
+                                                        const float* __restrict__ snz,
+                                                        const int64_t* __restrict__ ixyzso,
+                                                        const V2<float>* __restrict__ qstr,
+                                                        const V2<float>* __restrict__ mstr, int64_t ldstr,
+                                                        const int64_t* __restrict__ row_ptr,
+                                                        const int* __restrict__ near_sorted, float kr, float krev,
+                                                        float ki, int64_t chunk, V2<float>* __restrict__ part,
+                                                        int64_t ldpart) {
+  __shared__ float4 geo[FT * 2];   // x y z nx | ny nz - -
+  __shared__ float4 str[FT * 16];  // (q re, q im, m re, m im) of source j, right-hand side r at j * 16 + r
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int rg = blockIdx.z;
+  qstr += (int64_t)rg * 16 * ldstr;
+  mstr += (int64_t)rg * 16 * ldstr;
+  const int64_t t0 = (blockIdx.x * (int64_t)FW + wv) * 16;
+  const int64_t i = t0 + (lane & 15);  // this lane's target
+  const bool act = i < ntarg;
+  const float x = act ? tx[i] : 0.0f, y = act ? ty[i] : 0.0f, z = act ? tz[i] : 0.0f;
+  const int ks = lane >> 4;  // this lane's source within a step of 4
+  f32x4 are = {0.0f, 0.0f, 0.0f, 0.0f}, aim = {0.0f, 0.0f, 0.0f, 0.0f};
+  const int64_t jbeg = blockIdx.y * chunk;
+  const int64_t jend = jbeg + chunk < nsrc ? jbeg + chunk : nsrc;
+  constexpr int64_t NONE = INT64_MAX;
+  int64_t nptr = 0, nend = 0, ea = NONE, eb = NONE;
+  auto advance = [&](int64_t j) {
+    ea = NONE;
+    eb = NONE;
+    while (nptr < nend) {
+      const int p = near_sorted[nptr];
+      const int64_t e = ixyzso[p + 1] - 1;
+      if (e > j) {
+        ea = ixyzso[p] - 1;
+        eb = e;
+        break;
+      }
+      nptr++;
+    }
+  };
+  if (act && jbeg < jend) {
+    nptr = row_ptr[i] - 1;
+    nend = row_ptr[i + 1] - 1;
+    advance(jbeg);
+  }
+  for (int64_t base = jbeg; base < jend; base += FT) {
+    const int cnt = jend - base < FT ? (int)(jend - base) : FT;
+    __syncthreads();
+    for (int e = threadIdx.x; e < FT; e += 64 * FW) {
+      const int64_t j = base + e;
+      const bool ok = e < cnt;
+      geo[2 * e] = ok ? make_float4(sx[j], sy[j], sz[j], snx[j]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      geo[2 * e + 1] = ok ? make_float4(sny[j], snz[j], 0.0f, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    // strengths: consecutive threads read consecutive sources of one right-hand side
+    for (int e = threadIdx.x; e < FT * 16; e += 64 * FW) {
+      const int jl = e % FT, r = e / FT;
+      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (jl < cnt) {
+        const V2<float> q = qstr[r * ldstr + base + jl], m = mstr[r * ldstr + base + jl];
+        v = make_float4(q.x, q.y, m.x, m.y);
+      }
+      str[jl * 16 + r] = v;
+    }
+    __syncthreads();
+    // sources past the tile's count carry zero strengths (and are masked below)
+    for (int k = 0; k < cnt; k += 4) {
+      const int jl = k + ks;
+      const int64_t jk = base + jl;
+      if (jk >= eb) advance(jk);
+      const float4 g0 = geo[2 * jl], g1 = geo[2 * jl + 1];
+      const float dx = x - g0.x, dy = y - g0.y, dz = z - g0.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      float rinv = __frsqrt_rn(r2);
+      rinv = (act && jl < cnt && jk < ea) ? rinv : 0.0f;
+      const float d = dx * g0.w + dy * g1.x + dz * g1.y;
+      float c, sn, tdr, tdi;
+      pair_factors<float, GEN>(r2, rinv, d, kr, krev, ki, c, sn, tdr, tdi);
+      const float gr = c * rinv, gi = sn * rinv;
+      const float hr = tdr * gr - tdi * gi, hi = tdr * gi + tdi * gr;
+      const float4 sv = str[(k + ks) * 16 + (lane & 15)];  // source k + l / 16, right-hand side l % 16
+      are = __builtin_amdgcn_mfma_f32_16x16x4f32(gr, sv.x, are, 0, 0, 0);
+      are = __builtin_amdgcn_mfma_f32_16x16x4f32(gi, -sv.y, are, 0, 0, 0);
+      are = __builtin_amdgcn_mfma_f32_16x16x4f32(hr, sv.z, are, 0, 0, 0);
+      are = __builtin_amdgcn_mfma_f32_16x16x4f32(hi, -sv.w, are, 0, 0, 0);
+      aim = __builtin_amdgcn_mfma_f32_16x16x4f32(gr, sv.y, aim, 0, 0, 0);
+      aim = __builtin_amdgcn_mfma_f32_16x16x4f32(gi, sv.x, aim, 0, 0, 0);
+      aim = __builtin_amdgcn_mfma_f32_16x16x4f32(hr, sv.w, aim, 0, 0, 0);
+      aim = __builtin_amdgcn_mfma_f32_16x16x4f32(hi, sv.z, aim, 0, 0, 0);
+    }
+  }
+  // accumulator element r of lane l: target 4 (l / 16) + r, right-hand side l % 16
+  const int rhs = lane & 15;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int64_t it = t0 + 4 * (lane >> 4) + r;
+    if (it < ntarg) part[(int64_t)blockIdx.y * ldpart + (int64_t)(rg * 16 + rhs) * ntarg + it] = {are[r], aim[r]};
+  }
+}
+
 // pot[r][i] = (sum over source splits of part) / (4 pi)
 template <typename T>
 __global__ void k_farsum_reduce(int64_t ntarg, int nrhs, int nsplit, const V2<T>* __restrict__ part,
@@ -767,6 +885,38 @@ static int launch_far(ApplyPlan* P, bool mask, int64_t wg_target, const V2<T>* q
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// The matrix-core far sum: 16 right-hand sides of one group per launch column, FP32,
+// near patches masked; 64 targets per workgroup, the source range split as for k_farsum.
+static int launch_far_mfma(ApplyPlan* P, int64_t wg_target, const V2<float>* q, const V2<float>* m,
+                           int64_t ldstr, int nrhs, V2<float>* part, int64_t part_cap, int64_t* nsplit_out,
+                           hipStream_t s) {
+  const float* const* src = (const float* const*)P->src32;
+  const float* const* trg = (const float* const*)P->trg32;
+  const float kr = (float)(P->helm ? P->zk[0] : 0.0), ki = (float)(P->helm ? P->zk[1] : 0.0);
+  const float krev = (float)(P->helm ? P->zk[0] * 0.15915494309189533577 : 0.0);  // k / (2 pi)
+  const unsigned g = nblk(P->ntarg, 16 * FW);
+  const int ngrp = nrhs / 16;
+  const int64_t ntiles = (P->nptso + FT - 1) / FT;
+  int64_t nsplit = (wg_target + (int64_t)g * ngrp - 1) / ((int64_t)g * ngrp);
+  const int64_t cap = part_cap / ((int64_t)nrhs * (P->ntarg > 0 ? P->ntarg : 1));
+  if (nsplit > cap) nsplit = cap;
+  if (nsplit > ntiles) nsplit = ntiles;
+  if (nsplit < 1) nsplit = 1;
+  const int64_t chunk = ((ntiles + nsplit - 1) / nsplit) * FT;
+  nsplit = (P->nptso + chunk - 1) / chunk;
+  const int64_t ldpart = (int64_t)nrhs * P->ntarg;
+  dim3 grid(g, (unsigned)nsplit, (unsigned)ngrp);
+#define FMM_FARM(CKV)                                                                                          \
+  k_farsum_mfma<CKV><<<grid, 64 * FW, 0, s>>>(P->ntarg, trg[0], trg[1], trg[2], P->nptso, src[0], src[1], src[2], \
+                                             src[3], src[4], src[5], P->ixyzso, q, m, ldstr, P->row_ptr,         \
+                                             P->near_sorted, kr, krev, ki, chunk, part, ldpart)
+  if (ki != 0.0f || !P->phase_small) FMM_FARM(true);
+  else FMM_FARM(false);
+#undef FMM_FARM
+  *nsplit_out = nsplit;
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 template <typename T>
 static int far_group(const ApplyPlan* P, int nrhs, int prec, int64_t* wg) {
   const auto it = P->tuned.find(prec * 1000 + nrhs);
@@ -775,6 +925,7 @@ static int far_group(const ApplyPlan* P, int nrhs, int prec, int64_t* wg) {
     return it->second.first;
   }
   *wg = P->far_wg_target;
+  if (sizeof(T) == 4 && nrhs % 16 == 0) return 16;  // the matrix-core far sum
   const int pref = sizeof(T) == 8 ? 4 : 8;
   for (int gsz = pref; gsz > 1; gsz >>= 1)
     if (nrhs % gsz == 0) return gsz;
@@ -790,8 +941,10 @@ static int dispatch_far(ApplyPlan* P, bool mask, int grp, int64_t wg, const V2<T
     case 4: return launch_far<T, 4>(P, mask, wg, q, m, ldstr, nrhs, part, part_cap, nsplit, s);
     case 8: return launch_far<T, 8>(P, mask, wg, q, m, ldstr, nrhs, part, part_cap, nsplit, s);
     case 16:
-      if constexpr (sizeof(T) == 4)
+      if constexpr (sizeof(T) == 4) {
+        if (mask) return launch_far_mfma(P, wg, q, m, ldstr, nrhs, part, part_cap, nsplit, s);
         return launch_far<T, 16>(P, mask, wg, q, m, ldstr, nrhs, part, part_cap, nsplit, s);
+      }
       return -6;
     default: return -6;
   }

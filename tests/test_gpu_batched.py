@@ -100,3 +100,26 @@ def test_fp32_far_sum_phase_beyond_hardware_sine_range(cuda):
     p32 = plan.apply(dS.to(torch.complex64), 32, 1).cpu().numpy()
     assert _rel(p32, p64) < 2e-4, _rel(p32, p64)
     plan.close()
+
+
+@pytest.mark.parametrize("zk,tol", [(5.0, 1e-5), (400.0, 2e-4), (3.0 + 1.5j, 1e-5)])
+def test_matrix_core_far_sum_matches_fp64(cuda, zk, tol):
+    """The FP32 far sum of 16 right-hand sides on the matrix cores (plain phases, phases
+    beyond the hardware sine's range, a complex wavenumber) against the FP64 apply, one
+    right-hand side at a time; 32 right-hand sides run as two groups."""
+    import torch
+    from fmm3dbie_amd import geometry as G, layer, problem
+    g = G.get_sphere_npat(1.0, 2, (0, 0, 0), 4)
+    pars = np.array([zk, 1j * zk, 1.0])
+    P = problem.setup(g, 1e-6, True, pars)
+    w = np.zeros(P["nquad"], complex)
+    plan = layer.ApplyPlan(g, True, pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"], P["ixyzso"],
+                           P["srcover"], P["whtsover"])
+    rng = np.random.default_rng(11)
+    N = len(g["srcvals"])
+    dS = torch.as_tensor(rng.standard_normal((32, N)) + 1j * rng.standard_normal((32, N)), device=cuda)
+    ref = np.stack([plan.apply(dS[r], 64, 1).cpu().numpy() for r in range(32)])
+    p32 = plan.apply(dS.to(torch.complex64), 32, 1).cpu().numpy()
+    for r in range(32):
+        assert _rel(p32[r], ref[r]) < tol, (r, _rel(p32[r], ref[r]))
+    plan.close()
