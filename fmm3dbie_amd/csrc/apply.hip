@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
 // sides.  Accumulator element r of lane l is target 4 (l / 16) + r, right-hand side l % 16.
 // The workgroup stages FT sources at a time in LDS: geometry (x, y, z, nx, ny, nz) and
 // the (q, m) of the 16 right-hand sides of each source.
-constexpr int FT = 128;        // sources per LDS tile
+constexpr int FT = 64;         // sources per LDS tile (18 KB of LDS per workgroup with FW = 4)
 constexpr int FW = 4;          // wavefronts per workgroup (16 targets each)
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 template <bool GEN>
