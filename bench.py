@@ -117,7 +117,7 @@ def main():
     ap.add_argument("--batched-sequential", type=int, default=8,
                     help="FP64 single-RHS solves timed for the batched comparison")
     ap.add_argument("--apply-only", action="store_true",
-                    help="only the near quadrature once, then 5 FP64 (add/subtract) and 5 FP32 (masked) applies "
+                    help="only the near quadrature once, then 5 FP64 (add/subtract), 5 FP32 (masked) and 5 16-RHS FP32 applies "
                          "of the workload (for the apply's rocprofv3 counters)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device read + 1 GiB write (torch.neg) for HBM counter scaling")
@@ -383,9 +383,10 @@ def _timed(fn):
 
 
 def apply_only(g, helm, pars, P, nq, dev):
-    """5 FP64 applies in the reference's add/subtract algebra and 5 FP32 masked applies of
-    a random density, with the per-phase HIP-event times (ms): oversampling, far sum,
-    split sum + CSR correction, subtraction."""
+    """5 FP64 applies in the reference's add/subtract algebra, 5 FP32 masked applies of a
+    random density and 5 FP32 masked applies of 16 random densities (the far sum on the
+    matrix cores), with the per-phase HIP-event times (ms): oversampling, far sum, split
+    sum + CSR correction, subtraction."""
     import torch
     from fmm3dbie_amd import layer
     w = nq.run()
@@ -395,7 +396,10 @@ def apply_only(g, helm, pars, P, nq, dev):
     rng = np.random.default_rng(0)
     sig = torch.as_tensor(rng.standard_normal(npts) + 1j * rng.standard_normal(npts), device=dev)
     out = {"workload_points": npts, "oversampled_sources": nptso, "nnz": P["nnz"], "nquad": P["nquad"]}
-    for name, prec, mode, s in (("fp64_addsub", 64, 0, sig), ("fp32_masked", 32, 1, sig.to(torch.complex64))):
+    sig16 = torch.as_tensor(rng.standard_normal((16, npts)) + 1j * rng.standard_normal((16, npts)),
+                            device=dev).to(torch.complex64)
+    for name, prec, mode, s in (("fp64_addsub", 64, 0, sig), ("fp32_masked", 32, 1, sig.to(torch.complex64)),
+                                ("fp32_masked_16rhs", 32, 1, sig16)):
         tm = np.zeros(4)
         acc = []
         plan.apply(s, prec, mode)

@@ -297,10 +297,12 @@ __global__ void __launch_bounds__(TPB) k_farsum(int64_t ntarg, const T* __restri
 // potentials are the complex products pot = G q + H m of the target x source kernel
 // matrices with the source x 16 strength matrices.  A wavefront takes 16 targets; in a
 // step of 4 sources lane l evaluates the pair (target l % 16, source l / 16) once, and
-// v_mfma_f32_16x16x4f32 multiplies the 16 x 4 blocks Re G, Im G, Re H, Im H by the
-// 4 x 16 strength blocks (lane l supplies source l / 16, right-hand side l % 16): eight
-// products per step give the real and imaginary parts of 16 targets x 16 right-hand
-// sides.  Accumulator element r of lane l is target 4 (l / 16) + r, right-hand side l % 16.
+// v_mfma_f32_16x16x4f32 multiplies 16 x 4 kernel blocks by 4 x 16 strength blocks (lane
+// l supplies source l / 16, right-hand side l % 16).  Each complex product takes three
+// real ones (Gauss): a1 = Re G Re q + Re H Re m, a2 = Im G Im q + Im H Im m,
+// a3 = (Re G + Im G)(Re q + Im q) + (Re H + Im H)(Re m + Im m), so Re pot = a1 - a2 and
+// Im pot = a3 - a1 - a2: six matrix products per step instead of eight.  Accumulator
+// element r of lane l is target 4 (l / 16) + r, right-hand side l % 16.
 // The workgroup stages FT sources at a time in LDS: geometry (x, y, z, nx, ny, nz) and
 // the (q, m) of the 16 right-hand sides of each source.
 constexpr int FT = 64;         // sources per LDS tile (18 KB of LDS per workgroup with FW = 4)
@@ -331,7 +333,7 @@ __global__ void __launch_bounds__(64 * FW) k_farsum_mfma(int64_t ntarg, const fl
   const bool act = i < ntarg;
   const float x = act ? tx[i] : 0.0f, y = act ? ty[i] : 0.0f, z = act ? tz[i] : 0.0f;
   const int ks = lane >> 4;  // this lane's source within a step of 4
-  f32x4 are = {0.0f, 0.0f, 0.0f, 0.0f}, aim = {0.0f, 0.0f, 0.0f, 0.0f};
+  f32x4 a1 = {0.0f, 0.0f, 0.0f, 0.0f}, a2 = {0.0f, 0.0f, 0.0f, 0.0f}, a3 = {0.0f, 0.0f, 0.0f, 0.0f};
   const int64_t jbeg = blockIdx.y * chunk;
   const int64_t jend = jbeg + chunk < nsrc ? jbeg + chunk : nsrc;
   constexpr int64_t NONE = INT64_MAX;
@@ -391,14 +393,12 @@ __global__ void __launch_bounds__(64 * FW) k_farsum_mfma(int64_t ntarg, const fl
       const float gr = c * rinv, gi = sn * rinv;
       const float hr = tdr * gr - tdi * gi, hi = tdr * gi + tdi * gr;
       const float4 sv = str[(k + ks) * 16 + (lane & 15)];  // source k + l / 16, right-hand side l % 16
-      are = __builtin_amdgcn_mfma_f32_16x16x4f32(gr, sv.x, are, 0, 0, 0);
-      are = __builtin_amdgcn_mfma_f32_16x16x4f32(gi, -sv.y, are, 0, 0, 0);
-      are = __builtin_amdgcn_mfma_f32_16x16x4f32(hr, sv.z, are, 0, 0, 0);
-      are = __builtin_amdgcn_mfma_f32_16x16x4f32(hi, -sv.w, are, 0, 0, 0);
-      aim = __builtin_amdgcn_mfma_f32_16x16x4f32(gr, sv.y, aim, 0, 0, 0);
-      aim = __builtin_amdgcn_mfma_f32_16x16x4f32(gi, sv.x, aim, 0, 0, 0);
-      aim = __builtin_amdgcn_mfma_f32_16x16x4f32(hr, sv.w, aim, 0, 0, 0);
-      aim = __builtin_amdgcn_mfma_f32_16x16x4f32(hi, sv.z, aim, 0, 0, 0);
+      a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(gr, sv.x, a1, 0, 0, 0);
+      a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(hr, sv.z, a1, 0, 0, 0);
+      a2 = __builtin_amdgcn_mfma_f32_16x16x4f32(gi, sv.y, a2, 0, 0, 0);
+      a2 = __builtin_amdgcn_mfma_f32_16x16x4f32(hi, sv.w, a2, 0, 0, 0);
+      a3 = __builtin_amdgcn_mfma_f32_16x16x4f32(gr + gi, sv.x + sv.y, a3, 0, 0, 0);
+      a3 = __builtin_amdgcn_mfma_f32_16x16x4f32(hr + hi, sv.z + sv.w, a3, 0, 0, 0);
     }
   }
   // accumulator element r of lane l: target 4 (l / 16) + r, right-hand side l % 16
@@ -406,7 +406,8 @@ __global__ void __launch_bounds__(64 * FW) k_farsum_mfma(int64_t ntarg, const fl
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int64_t it = t0 + 4 * (lane >> 4) + r;
-    if (it < ntarg) part[(int64_t)blockIdx.y * ldpart + (int64_t)(rg * 16 + rhs) * ntarg + it] = {are[r], aim[r]};
+    if (it < ntarg)
+      part[(int64_t)blockIdx.y * ldpart + (int64_t)(rg * 16 + rhs) * ntarg + it] = {a1[r] - a2[r], a3[r] - a1[r] - a2[r]};
   }
 }
 

@@ -1,5 +1,6 @@
-// Learned refinement-depth selector: an 8 -> 64 -> 64 -> 6 MLP (ReLU, softmax
-// cross-entropy) trained on the GPU on the adaptive integrator's own refinement counts.
+// Learned order selection: an 8 -> 64 -> 64 -> 6 MLP (ReLU, softmax cross-entropy)
+// trained on the GPU to predict each adaptive pair's rule class (the XG degree of least
+// cost, labelled from the adaptive integrator's own runs).
 // Parameters (FP32, packed): W1[64][8] b1[64] W2[64][64] b2[64] W3[6][64] b3[6].
 //
 // Training step = k_mlp_grad (64 samples per workgroup: forward, backward, per-workgroup

@@ -1,4 +1,4 @@
-// Learned refinement-depth selector (8 -> 64 -> 64 -> 6 MLP, FP32) on the device.
+// Learned order selection (8 -> 64 -> 64 -> 6 MLP, FP32) on the device.
 #pragma once
 #include <hip/hip_runtime.h>
 
