@@ -489,13 +489,25 @@ constexpr int NCACHE_TRI = 341;
 constexpr int CACHE_STRIDE = 8;
 constexpr int NCACHE_PARENT = 85;  // triangles whose children can be cached (depths 0..3)
 
-// Rows of the cached-tree basis table are padded to an even length, so every row starts on
-// a 16-byte boundary and is read with 16-byte loads, and to a length that is not a
-// multiple of 8 doubles: the adaptive kernel's lanes write their rows of the same layout
-// in LDS, and a row stride of a multiple of 64 bytes would put the lanes' writes into
-// few banks (order 4, 15 basis functions: rows of 18 instead of 16 doubles, 5% faster).
+// Row q of the cached-tree basis table: two slots for node q's kernel value (zero in the
+// table; the adaptive kernel writes the value there in its LDS copy of the row), then
+// P_0..P_{NP-1}.  Rows are padded to an even length, so every row starts on a 16-byte
+// boundary and is read with 16-byte loads, and to a length that is not a multiple of 8
+// doubles: the adaptive kernel's lanes write their rows of the same layout in LDS, and a
+// row stride of a multiple of 64 bytes would put the lanes' writes into few banks (order
+// 4, 15 basis functions: rows of 18 instead of 16 doubles, 5% faster).
 __host__ __device__ constexpr int ptab_w(int np) {
-  return ((np + 1) & ~1) % 8 == 0 ? ((np + 1) & ~1) + 2 : ((np + 1) & ~1);
+  return row_w(np) % 8 == 0 ? row_w(np) + 2 : row_w(np);
+}
+
+// Nodes per chunk of a refinement step in the adaptive kernel, and the LDS of a chunk:
+// whole 1024-byte blocks (one load-to-LDS instruction each) covering chunk_rows rows.
+// At order 6, 55 rows of 30 doubles take 13 blocks (13 KB): 12 wavefronts per CU, where
+// 64 rows (15 KB) allow 10; the order-6 plain rule of 55 nodes then fills one chunk per
+// child.
+__host__ __device__ constexpr int chunk_rows(int n) { return n >= 6 ? 55 : WAVE; }
+__host__ __device__ constexpr int chunk_blocks(int n) {
+  return (chunk_rows(n) * ptab_w((n + 1) * (n + 2) / 2) * 8 + 1023) / 1024;
 }
 
 __host__ __device__ __forceinline__ int cached_child(int t, int c) {
@@ -526,7 +538,7 @@ __global__ void k_tree_cache(int64_t npatches, int kq, int ntri, const double* _
     double g[12];
     for (int m = 0; m < 9; m++) g[m] = 0.0;
     for (int l = 0; l < NP; l++) {
-      const double pl = ptab[(size_t)q * ptab_w(NP) + l];
+      const double pl = ptab[(size_t)q * ptab_w(NP) + 2 + l];
       for (int m = 0; m < 9; m++) g[m] = fma(sc[9 * l + m], pl, g[m]);
     }
     const double jac = geom_finish(g);
@@ -560,10 +572,10 @@ struct AdapRules {
 };
 
 // Column sums of staged rows r0 <= r < r1 (no barriers: the caller brackets the chunk):
-// row q holds the basis values P_0..P_{NP-1} of node q at pb + q * PW and its kernel value
-// f_q at fb[q].
+// row q at pb + q * PW holds the kernel value f_q of node q (re, im) and its basis values
+// P_0..P_{NP-1}.
 template <int NP, bool HELM>
-__device__ __forceinline__ Mom column_sum_rows(const double* pb, const double2* fb, int r0, int r1) {
+__device__ __forceinline__ Mom column_sum_rows(const double* pb, int r0, int r1) {
   constexpr int PW = ptab_w(NP);
   constexpr int G = WAVE / NP;
   const int lane = threadIdx.x;
@@ -572,8 +584,8 @@ __device__ __forceinline__ Mom column_sum_rows(const double* pb, const double2* 
   if (grp < G) {
 #pragma unroll 4
     for (int q = r0 + grp; q < r1; q += G) {
-      const double p = pb[q * PW + j];
-      const double2 f = fb[q];
+      const double p = pb[q * PW + 2 + j];
+      const double2 f = *reinterpret_cast<const double2*>(pb + q * PW);
       sr = fma(f.x, p, sr);
       if (HELM) si = fma(f.y, p, si);
     }
@@ -589,17 +601,19 @@ __device__ __forceinline__ Mom column_sum_rows(const double* pb, const double2* 
 
 // Moments of the triangles of one refinement step (root: the patch triangle, m[0];
 // otherwise the four children of the triangle (x0, y0, dx, dy) at tree id tid, m[0..3]):
-// their ntri * kq nodes are spread over the lanes 64 at a time, so a small rule still
-// fills the wavefront and the kernel evaluations of all children are in flight together.
-// Children of a cached triangle have consecutive tree ids, so a chunk's basis rows are
-// one contiguous block of the basis table: it is copied to LDS by the memory unit
-// (64 x 16 bytes per load-to-LDS instruction, no registers) while the lanes read the
-// nodes' geometry from the tree cache and evaluate the kernel.  Below the cached depths
+// their ntri * kq nodes are spread over the lanes in chunks of at most chunk_rows(N)
+// nodes, of equal size, so a small rule still fills the wavefront and the kernel
+// evaluations of all children are in flight together.  Children of a cached triangle
+// have consecutive tree ids, so a chunk's rows are one contiguous block of the basis
+// table: it is copied to LDS by the memory unit (64 x 16 bytes per load-to-LDS
+// instruction, no registers) while the lanes read the nodes' geometry from the tree
+// cache and evaluate the kernel; each lane then puts its node's kernel value into the
+// row's first two slots.  Below the cached depths
 // each lane writes its node's basis row and sums the geometry from it and the patch
 // coefficients sc in LDS.  CACHED selects the path at compile time (the caller tests the
 // step's tree id), so each path is scheduled without the other's registers.
 template <int N, bool HELM, bool CACHED>
-__device__ __forceinline__ void step_moments(double* pb, double2* fb, const double* sc, bool root, int npar,
+__device__ __forceinline__ void step_moments(double* pb, const double* sc, bool root, int npar,
                                              int tid, double x0, double y0, double dx, double dy, int kq,
                                              const double* __restrict__ xg, const double* __restrict__ cpatch,
                                              const double* __restrict__ ptab, const double* tg, const KPar& kp,
@@ -615,18 +629,24 @@ __device__ __forceinline__ void step_moments(double* pb, double2* fb, const doub
   const double hx = 0.5 * dx, hy = 0.5 * dy, da = hx * hx;
 #pragma unroll
   for (int c = 0; c < 4; c++) m[c] = Mom{0.0, 0.0};
-  for (int base = 0; base < total; base += WAVE) {
+  constexpr int CHM = chunk_rows(N);
+  const int nch = (total + CHM - 1) / CHM;
+  const int ch = (total + nch - 1) / nch;  // rows per chunk (the last one possibly fewer)
+  for (int base = 0; base < total; base += ch) {
     const int n = base + lane;
+    const int nrows = total - base < ch ? total - base : ch;
     if (cached) {
-      // rows t0 kq + base .. + 63 of the table (padded by 64 rows past the last triangle)
+      // chunk_blocks(N) KB of the table from row t0 kq + base: the chunk's rows and a
+      // part of the next ones (the table is padded by 64 rows past its last triangle)
       const double* blk = ptab + ((size_t)t0 * kq + base) * PW;
 #pragma unroll
-      for (int k = 0; k < PW / 2; k++)
+      for (int k = 0; k < chunk_blocks(N); k++)
         __builtin_amdgcn_global_load_lds(
             (const __attribute__((address_space(1))) void*)(blk + 2 * (k * WAVE + lane)),
             (__attribute__((address_space(3))) void*)(pb + 2 * k * WAVE), 16, 0, 0);
     }
-    if (n < total) {
+    double2 fv = make_double2(0.0, 0.0);
+    if (lane < nrows) {
       const int c = ntri == 1 ? 0 : (n >= kq) + (n >= 2 * kq) + (n >= 3 * kq);
       const int q = n - c * kq;
       double g[12], w;
@@ -644,16 +664,17 @@ __device__ __forceinline__ void step_moments(double* pb, double2* fb, const doub
         const double cx = (c & 1) ? x0 + hx : x0, cy = (c & 2) ? y0 + hy : y0;
         const double cdx = c == 3 ? -hx : hx, cdy = c == 3 ? -hy : hy;
         const double u = cx + cdx * xg[3 * q], v = cy + cdy * xg[3 * q + 1];
-        koorn_geom_row<N>(u, v, sc, g, pb + lane * PW);
+        koorn_geom_row<N>(u, v, sc, g, pb + lane * PW + 2);
         w = geom_finish(g) * da * xg[3 * q + 2];
       }
       double vr, vi;
       kern_eval<HELM>(tg, g, kp, vr, vi);
-      fb[lane] = make_double2(vr * w, vi * w);
+      fv = make_double2(vr * w, vi * w);
     }
-    const int nrows = total - base < WAVE ? total - base : WAVE;
-    // the loads to LDS are counted by vmcnt: wait for them before the barrier
+    // the loads to LDS (counted by vmcnt) also fill the rows' kernel-value slots: wait
+    // for them before the lanes write their values there
     __builtin_amdgcn_s_waitcnt(0x0F70);
+    if (lane < nrows) *reinterpret_cast<double2*>(pb + lane * PW) = fv;
     __syncthreads();
     // the rows of this chunk belong to triangles c0..c1 (wave-uniform)
     const int c0 = ntri == 1 ? 0 : base / kq, c1 = ntri == 1 ? 0 : (base + nrows - 1) / kq;
@@ -662,7 +683,7 @@ __device__ __forceinline__ void step_moments(double* pb, double2* fb, const doub
       if (c < c0 || c > c1) continue;
       const int r0 = c * kq - base > 0 ? c * kq - base : 0;
       const int r1 = (c + 1) * kq - base < nrows ? (c + 1) * kq - base : nrows;
-      m[c] += column_sum_rows<NP, HELM>(pb, fb, r0, r1);
+      m[c] += column_sum_rows<NP, HELM>(pb, r0, r1);
     }
     __syncthreads();
   }
@@ -695,8 +716,7 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
                                                  int* __restrict__ overflow, double* __restrict__ wnear) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
   constexpr int ES = 6 + 2 * NP;  // stack entry: x0,y0,dx,dy,level,tree id, moments
-  __shared__ __align__(16) double pb[WAVE * ptab_w(NP)];
-  __shared__ __align__(16) double2 fb[WAVE];
+  __shared__ __align__(16) double pb[chunk_blocks(N) * 128];
   const int lane = threadIdx.x;
   double* stack = stack_all + (size_t)blockIdx.x * STACK_CAP * ES;
   const int nl = nlist_dev ? *nlist_dev : nlist;
@@ -741,7 +761,7 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
     const int npar = R.npar[cls];
     const double* cpatch = R.cache[cls] + (size_t)ip * CACHE_STRIDE * cache_tris(npar) * kq;
     Mom mc[4];
-    step_moments<N, HELM, true>(pb, fb, lsc, true, npar, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
+    step_moments<N, HELM, true>(pb, lsc, true, npar, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
     if (lane < 6) stack[lane] = lane < 2 ? 0.0 : (lane < 4 ? 1.0 : 0.0);  // level 0, tree id 0
     if (lane < NP) {
       stack[6 + lane] = mc[0].re;
@@ -759,9 +779,9 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
       const int tid = __builtin_amdgcn_readfirstlane((int)e[5]);  // cached tree id, -1 below depth 2
       const Mom mpar = lane < NP ? Mom{e[6 + lane], e[6 + NP + lane]} : Mom{0.0, 0.0};
       if (tid >= 0 && tid < npar)
-        step_moments<N, HELM, true>(pb, fb, lsc, false, npar, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
+        step_moments<N, HELM, true>(pb, lsc, false, npar, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
       else
-        step_moments<N, HELM, false>(pb, fb, lsc, false, npar, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
+        step_moments<N, HELM, false>(pb, lsc, false, npar, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
       nref++;
       nunc += (tid < 0 || tid >= npar) ? 1 : 0;  // children below the cached tree
       Mom ssum = mc[0];
@@ -1223,9 +1243,9 @@ T* upload(const std::vector<T>& h, hipStream_t s) {
   return d;
 }
 
-// Node tables of the cached tree (depths 0..3: 85 triangles) for the rule (xq, xw):
+// Node tables of the cached tree (depths 0..4: 341 triangles) for the rule (xq, xw):
 // (u, v) and area-scaled weight of every node and the basis values there (patch
-// independent).
+// independent; rows of ptab_w(NP) doubles, the basis from slot 2).
 template <int N>
 void cached_tree_tables(const std::vector<double>& xq, const std::vector<double>& xw, std::vector<double>& tuv,
                         std::vector<double>& twq, std::vector<double>& ptab) {
@@ -1234,7 +1254,8 @@ void cached_tree_tables(const std::vector<double>& xq, const std::vector<double>
   const int ncn = NCACHE_TRI * kq;
   tuv.assign(2 * ncn, 0.0);
   twq.assign(ncn, 0.0);
-  // 64 rows of padding: the adaptive kernel copies whole 64-row blocks of the table
+  // 64 rows of padding: the adaptive kernel copies whole 1024-byte blocks of the table
+  // from the first row of a chunk, up to chunk_blocks KB
   ptab.assign((size_t)(ncn + WAVE) * ptab_w(NP), 0.0);
   std::vector<double> pols(NP);
   struct Tri { double x0, y0, dx, dy; };
@@ -1256,7 +1277,7 @@ void cached_tree_tables(const std::vector<double>& xq, const std::vector<double>
       tuv[2 * n + 1] = tris[t].y0 + tris[t].dy * xq[2 * q + 1];
       twq[n] = da * xw[q];
       fmmhost::koorn_pols(tuv[2 * n], tuv[2 * n + 1], N, pols.data());
-      for (int l = 0; l < NP; l++) ptab[(size_t)n * ptab_w(NP) + l] = pols[l];
+      for (int l = 0; l < NP; l++) ptab[(size_t)n * ptab_w(NP) + 2 + l] = pols[l];
     }
   }
 }

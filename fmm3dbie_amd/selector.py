@@ -84,14 +84,20 @@ def init_params(seed=3):
     return p
 
 
+def chunk_rows(norder):
+    """Nodes per chunk of the adaptive kernel: 55 at order 6 and above (the LDS of 12
+    resident wavefronts per CU), else 64 (quad.hip chunk_rows)."""
+    return 55 if norder >= 6 else 64
+
+
 def collect(nq, stream=None):
     """Features of every adaptive pair (from a plain run) and its label: the rule class
     whose adaptive integration takes the least time.  Every pair is integrated once
     with each class (forced routing, no level cap, every kernel in order on one stream).
-    The kernel works in chunks of 64 nodes (a wavefront's lanes; the root is
-    ceil(kq/64) chunks, a refinement test's four children ceil(4kq/64)), and a chunk
-    costs the same whether its lanes are all busy or not, so the measured time of each
-    class's launch is fitted by a cost per chunk on the cached tree (depths 0..3), a
+    The kernel works in chunks of at most chunk_rows(norder) nodes (one node per lane;
+    the root is ceil(kq/c) chunks, a refinement test's four children ceil(4kq/c)), and a
+    chunk costs the same whether its lanes are all busy or not, so the measured time of
+    each class's launch is fitted by a cost per chunk on the cached tree, a
     cost per chunk below it (geometry evaluated) and a cost per pair, summed over the
     pairs' own counts; the label is the class of least fitted chunk cost among the
     classes worth their tree cache (open_classes).  Returns (X
@@ -130,7 +136,8 @@ def collect(nq, stream=None):
             unc = st["unc"][idx].astype(np.float64)
             ncached[c] = kq[c] * (1 + 4 * (nref - unc))
             ncomp[c] = kq[c] * 4 * unc
-            croot, cstep = -(-kq[c] // 64), -(-4 * kq[c] // 64)
+            cr = chunk_rows(nq.norder)
+            croot, cstep = -(-kq[c] // cr), -(-4 * kq[c] // cr)
             ccached[c] = croot + cstep * (nref - unc)
             ccomp[c] = cstep * unc
             lev[c] = st["lev"][idx]
