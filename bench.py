@@ -654,7 +654,7 @@ def cpu_baseline(g, helm, pars, eps, P, budget_s):
     os.environ["OMP_PROC_BIND"] = "close"
     os.environ["OMP_PLACES"] = "cores"
     from oracle import oracle as O
-    from fmm3dbie_amd.layer import SELF_N
+    from fmm3dbie_amd.layer import SELF_NR, SELF_NT, SELF_TAU
     threads = O.set_num_threads(usable)
     N = len(g["srcvals"])
     row_ptr = P["row_ptr"]
@@ -670,7 +670,7 @@ def cpu_baseline(g, helm, pars, eps, P, budget_s):
         iq[1:] += np.cumsum(np.diff(g["ixyzs"])[cols - 1])
         t0 = time.perf_counter()
         O.getnearquad(g, g["srcvals"][rows], P["ipatch_id"][rows], P["uvs_targ"][rows], eps, kind, pars, sub_ptr,
-                      cols, iq, P["rfac0"], O.SELF_POLAR, SELF_N, SELF_N)
+                      cols, iq, P["rfac0"], O.SELF_POLAR, SELF_NR, SELF_NT, self_tau=SELF_TAU)
         dt = time.perf_counter() - t0
         rate = len(cols) / dt
         used = len(cols)

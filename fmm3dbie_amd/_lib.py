@@ -25,7 +25,7 @@ SIGS = {
     "get_rfacs_": ([vp, vp, vp, vp], None),
     "fmm3dbie_last_error": ([], i32),
     "fmm3dbie_nearquad_dev": ([i32, i32, i32, i64, vp, vp, vp, i64, i64, vp, vp, vp, f64, vp, i64, vp, vp, vp,
-                               f64, vp, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
+                               f64, vp, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
     "fmm3dbie_mlp_nparam": ([], i32),
     "fmm3dbie_release_workspace": ([], None),
     "fmm3dbie_set_side_stream": ([C.c_int], None),

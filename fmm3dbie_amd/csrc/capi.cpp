@@ -107,8 +107,9 @@ int nearquad_host(bool helm, int64_t npatches, const int64_t* norders, const int
   if (rc == 0) {
     (void)hipMemsetAsync(dw, 0, wn * sizeof(double), s);
     A.wnear = dw;
-    A.self_n = 8;
-    A.self_tau_panel = 1.0;
+    A.self_nr = SELF_NR;
+    A.self_nt = SELF_NT;
+    A.self_tau_panel = SELF_TAU;
     rc = fmm_nearquad_device(A, nullptr, s);
   }
   if (rc == 0 && hipMemcpyAsync(wnear, dw, wn * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) rc = -1;
@@ -617,7 +618,8 @@ int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatc
                           const double* srccoefs, const double* srcvals, int64_t ndtarg, int64_t ntarg,
                           const double* targs, const int64_t* ipatch_id, const double* uvs_targ, double eps,
                           const double* kpar6, int64_t nnz, const int64_t* row_ptr, const int64_t* col_ind,
-                          const int64_t* iquad, double rfac0, double* wnear, int self_n, double self_tau_panel,
+                          const int64_t* iquad, double rfac0, double* wnear, int self_nr, int self_nt,
+                          double self_tau_panel,
                           void* stream, double* stats, int* nref_host, int* class_host, const float* mlp_params,
                           float* feat_out, int* pred_host, const int* route_lmax, int* lev_host,
                           int* unc_host) {
@@ -634,8 +636,9 @@ int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatc
   A.beta[0] = kpar6[4]; A.beta[1] = kpar6[5];
   A.nnz = nnz; A.row_ptr = row_ptr; A.col_ind = col_ind; A.iquad = iquad;
   A.rfac0 = rfac0; A.wnear = wnear;
-  A.self_n = self_n > 0 ? self_n : 8;
-  A.self_tau_panel = self_tau_panel > 0 ? self_tau_panel : 1.0;
+  A.self_nr = self_nr > 0 ? self_nr : SELF_NR;
+  A.self_nt = self_nt > 0 ? self_nt : SELF_NT;
+  A.self_tau_panel = self_tau_panel > 0 ? self_tau_panel : SELF_TAU;
   A.want_nref = nref_host != nullptr;
   A.nref_host = nref_host;
   A.class_host = class_host;

@@ -54,11 +54,12 @@ constexpr int NLMAX = 20;
 // keeps the GPU busy and the kernels beside it only take its slots.
 constexpr int64_t SIDE_MAX_PATCHES = 4096;
 
-struct GLRule {
-  double x[16];
-  double w[16];
+// Gauss-Legendre rules on [0, 1] of the self rule: radial (r) and sinh-parameter (t).
+struct SelfGL {
+  double xr[16], wr[16];
+  double xt[16], wt[16];
 };
-__constant__ GLRule g_self_gl;
+__constant__ SelfGL g_self_gl;
 
 template <bool HELM>
 __device__ __forceinline__ void kern_eval(const double* tg, const double* s, const KPar& kp, double& vr,
@@ -988,7 +989,7 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
       }
   }
   double cgl = 0.0;
-  for (int i = 0; i < nr; i++) cgl += g_self_gl.w[i] / g_self_gl.x[i];
+  for (int i = 0; i < nr; i++) cgl += g_self_gl.wr[i] / g_self_gl.xr[i];
   double pvs = 0.0;  // this lane's rays: sum of wtau (ray Jacobian) kappa (ln(R / rcut) - C_GL)
   const float inv_nr = 1.0f / (float)nr;
   // The three triangles at the target, each split at the foot of the perpendicular: up to
@@ -999,42 +1000,49 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
   __shared__ int segc[7];        // first node of each segment; segc[nseg] = node count
   __shared__ int segt[7];        // first ray (tau node) of each segment
   __shared__ double sh_rcut;
-  if (lane == 0) {
-    int ns = 0, cnt = 0, cntt = 0;
-    double hmin = 1e300;
-    for (int e = 0; e < 3; e++) {
-      const double Ax = vx[e] - u0, Ay = vy[e] - v0;
-      const double Bx = vx[(e + 1) % 3] - u0, By = vy[(e + 1) % 3] - v0;
-      const double ax = l11 * Ax + l12 * Ay, ay = l22 * Ay;
-      const double bx = l11 * Bx + l12 * By, by = l22 * By;
-      const double len = sqrt((bx - ax) * (bx - ax) + (by - ay) * (by - ay));
-      const double ex = (bx - ax) / len, ey = (by - ay) / len;
-      const double ts = -(ax * ex + ay * ey);
-      const double fx = ax + ts * ex, fy = ay + ts * ey;
-      const double d = sqrt(fx * fx + fy * fy);
-      hmin = d < hmin ? d : hmin;
-      const double s0 = -ts, s1 = len - ts;
-      double segs[3];
-      int nseg = 0;
-      segs[nseg++] = s0;
-      if (s0 < 0.0 && s1 > 0.0) segs[nseg++] = 0.0;
-      segs[nseg++] = s1;
-      for (int sg = 0; sg + 1 < nseg; sg++) {
-        const double ta = asinh(segs[sg] / d), tb = asinh(segs[sg + 1] / d);
-        const double span = fabs(tb - ta) / tau_panel;
-        int npan = span < 64.0 ? (int)ceil(span) : 64;  // also maps NaN to 1 below
-        if (!(npan >= 1)) npan = 1;
-        double* sd = segd[ns];
-        sd[0] = fx; sd[1] = fy; sd[2] = ex; sd[3] = ey; sd[4] = d; sd[5] = ta; sd[6] = tb; sd[7] = npan;
-        segt[ns] = cntt;
-        segc[ns++] = cnt;
-        cnt += npan * nn;
-        cntt += npan * nt;
-      }
+  // Lane 2e + k < 6 takes piece k of edge e: the edge parameter from its start to the
+  // foot (k = 0; to the end when the foot is outside the edge) or from the foot to the
+  // end (k = 1, when the foot splits the edge).  The pieces keep the order edge by edge.
+  bool has_piece = false;
+  if (lane < 6) {
+    const int e = lane >> 1, k = lane & 1;
+    const double Ax = vx[e] - u0, Ay = vy[e] - v0;
+    const double Bx = vx[(e + 1) % 3] - u0, By = vy[(e + 1) % 3] - v0;
+    const double ax = l11 * Ax + l12 * Ay, ay = l22 * Ay;
+    const double bx = l11 * Bx + l12 * By, by = l22 * By;
+    const double len = sqrt((bx - ax) * (bx - ax) + (by - ay) * (by - ay));
+    const double ex = (bx - ax) / len, ey = (by - ay) / len;
+    const double ts = -(ax * ex + ay * ey);
+    const double fx = ax + ts * ex, fy = ay + ts * ey;
+    const double d = sqrt(fx * fx + fy * fy);
+    const double s0 = -ts, s1 = len - ts;
+    const bool split = s0 < 0.0 && s1 > 0.0;
+    has_piece = k == 0 || split;
+    if (has_piece) {
+      const double sa = k == 0 ? s0 : 0.0, sb = (k == 0 && split) ? 0.0 : s1;
+      const double ta = asinh(sa / d), tb = asinh(sb / d);
+      const double span = fabs(tb - ta) / tau_panel;
+      int npan = span < 64.0 ? (int)ceil(span) : 64;  // also maps NaN to 1 below
+      if (!(npan >= 1)) npan = 1;
+      const uint64_t below = __ballot(1) & ((1ull << lane) - 1);  // active lanes: the pieces
+      double* sd = segd[__popcll(below)];
+      sd[0] = fx; sd[1] = fy; sd[2] = ex; sd[3] = ey; sd[4] = d; sd[5] = ta; sd[6] = tb; sd[7] = npan;
     }
-    segc[ns] = cnt;
-    segt[ns] = cntt;
-    for (int k = ns + 1; k < 7; k++) {
+  }
+  const int nseg = __popcll(__ballot(has_piece));
+  __syncthreads();
+  if (lane == 0) {
+    int cnt = 0, cntt = 0;
+    double hmin = 1e300;  // every edge has a piece k = 0
+    for (int s = 0; s < nseg; s++) {
+      const int npan = (int)segd[s][7];
+      hmin = segd[s][4] < hmin ? segd[s][4] : hmin;
+      segt[s] = cntt;
+      segc[s] = cnt;
+      cnt += npan * nn;
+      cntt += npan * nt;
+    }
+    for (int k = nseg; k < 7; k++) {
       segc[k] = cnt;
       segt[k] = cntt;
     }
@@ -1057,20 +1065,20 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
       double* row = buf + lane * row_w(NP);
       // kk < 256: a float reciprocal gives the exact quotient
       const int it = (int)(((float)kk + 0.5f) * inv_nr), ir = kk - it * nr;
-      const double tau = t0 + (t1 - t0) * g_self_gl.x[it];
+      const double tau = t0 + (t1 - t0) * g_self_gl.xt[it];
       // sinh and cosh from one exponential (only d * sinh enters, so the absolute error
       // of the difference near tau = 0 is what matters)
       const double et = exp(tau), eti = 1.0 / et;
       const double sh = 0.5 * (et - eti), ch = 0.5 * (et + eti);
-      const double wtau = g_self_gl.w[it] * (t1 - t0);
+      const double wtau = g_self_gl.wt[it] * (t1 - t0);
       const double px = fx + d * sh * ex, py = fy + d * sh * ey;
-      const double rho = g_self_gl.x[ir];
+      const double rho = g_self_gl.xr[ir];
       const double xi0 = rho * px, xi1 = rho * py;
       const double dv = xi1 / l22, du = (xi0 - l12 * dv) / l11;
       double g[12];
       koorn_geom_row<N>(u0 + du, v0 + dv, sc, g, row + 2);
       const double jac = geom_finish(g);
-      const double w = g_self_gl.w[ir] * wtau * rho * d * d * ch / detl * jac;
+      const double w = g_self_gl.wr[ir] * wtau * rho * d * d * ch / detl * jac;
       double vr, vi;
       kern_eval<HELM>(tg, g, kp, vr, vi);
       row[0] = vr * w;
@@ -1091,10 +1099,10 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
       const int loc = k - segt[sgi];
       const int ipan = loc / nt, it = loc - ipan * nt;
       const double t0 = ta + (tb - ta) * ipan / npan, t1 = ta + (tb - ta) * (ipan + 1) / npan;
-      const double tau = t0 + (t1 - t0) * g_self_gl.x[it];
+      const double tau = t0 + (t1 - t0) * g_self_gl.xt[it];
       const double et = exp(tau), eti = 1.0 / et;
       const double sh = 0.5 * (et - eti), ch = 0.5 * (et + eti);
-      const double wtau = g_self_gl.w[it] * (t1 - t0);
+      const double wtau = g_self_gl.wt[it] * (t1 - t0);
       const double px = fx + d * sh * ex, py = fy + d * sh * ey;
       const double dv0 = py / l22, du0 = (px - l12 * dv0) / l11;
       const double Dx = gx[0][0] * du0 + gx[1][0] * dv0, Dy = gx[0][1] * du0 + gx[1][1] * dv0,
@@ -1188,7 +1196,7 @@ std::map<std::tuple<int, int, int>, RuleTables> g_rules;
 
 // Constant-memory tables are per device: uploaded once per device (Koornwinder
 // constants) and when the self rule size changes (its Gauss-Legendre rule).
-std::map<int, int> g_const_dev;  // device -> self rule size last uploaded (0: none)
+std::map<int, int> g_const_dev;  // device -> self rule sizes last uploaded, 16 nr + nt (0: none)
 
 int ensure_constants() {
   int dev = 0;
@@ -1201,15 +1209,17 @@ int ensure_constants() {
   return 0;
 }
 
-int set_self_gl(int n) {
+int set_self_gl(int nr, int nt) {
   int dev = 0;
   HCHK(hipGetDevice(&dev));
+  const int key = 16 * nr + nt;
   auto it = g_const_dev.find(dev);
-  if (it != g_const_dev.end() && it->second == n) return 0;
-  GLRule gl = {};
-  fmmhost::gauss_legendre01(n, gl.x, gl.w);
+  if (it != g_const_dev.end() && it->second == key) return 0;
+  SelfGL gl = {};
+  fmmhost::gauss_legendre01(nr, gl.xr, gl.wr);
+  fmmhost::gauss_legendre01(nt, gl.xt, gl.wt);
   HCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_self_gl), &gl, sizeof(gl)));
-  g_const_dev[dev] = n;
+  g_const_dev[dev] = key;
   return 0;
 }
 
@@ -1521,7 +1531,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     HCHK(hipEventRecord(ev[12], ss));
     if (nself > 0) {
       k_self<N, HELM><<<nself, 64, 0, ss>>>(nself, d_self, A.col_ind, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
-                                            A.uvs_targ, A.iquad, d_umat, kp, A.self_n, A.self_n,
+                                            A.uvs_targ, A.iquad, d_umat, kp, A.self_nr, A.self_nt,
                                             A.self_tau_panel, A.wnear);
       HCHK(hipGetLastError());
     }
@@ -1755,11 +1765,11 @@ int dispatch(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
 int fmm_nearquad_device(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   std::lock_guard<std::mutex> lock(g_quad_mutex);
   if (ensure_constants() != 0) return -1;
-  if (set_self_gl(A.self_n) != 0) return -1;
-  if (A.self_n < 1 || A.self_n > 16) {
-    fprintf(stderr, "fmm3dbie_amd: self rule size %d out of range (1..16)\n", A.self_n);
+  if (A.self_nr < 1 || A.self_nr > 16 || A.self_nt < 1 || A.self_nt > 16) {
+    fprintf(stderr, "fmm3dbie_amd: self rule size %d x %d out of range (1..16)\n", A.self_nr, A.self_nt);
     return -5;
   }
+  if (set_self_gl(A.self_nr, A.self_nt) != 0) return -1;
   return A.helm ? dispatch<true>(A, st, s) : dispatch<false>(A, st, s);
 }
 

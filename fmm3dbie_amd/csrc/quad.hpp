@@ -3,6 +3,16 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Default self rule: panels of at most SELF_TAU in the sinh parameter, SELF_NR x SELF_NT
+// Gauss-Legendre points (radial x sinh parameter).  Against a 16 x 16 rule on panels of
+// 0.5, the combined-field weights of every target of sampled patches agree to 1.7e-9
+// (cube sphere order 4), 1.0e-8 (wavy torus order 6) and 2.3e-8 (deformed sphere order
+// 6) of the target's largest weight, as with 8 x 8 points on panels of 1, with 24% fewer
+// nodes.
+constexpr int SELF_NR = 8;
+constexpr int SELF_NT = 10;
+constexpr double SELF_TAU = 2.0;
+
 struct NearQuadArgs {
   bool helm;              // Helmholtz (complex weights) or Laplace (real weights)
   int norder;             // patch order (uniform over patches)
@@ -23,7 +33,7 @@ struct NearQuadArgs {
   const int64_t* iquad;    // device, 1-based
   double rfac0;
   double* wnear;           // device, nquad complex (interleaved) or real
-  int self_n;              // Gauss-Legendre points per direction of the self rule
+  int self_nr, self_nt;    // Gauss-Legendre points of the self rule's panels (radial, sinh parameter)
   double self_tau_panel;   // max sinh-parameter panel length of the self rule
   int adaptive_waves;      // persistent wavefronts of the adaptive kernel (0 = default)
   bool want_nref;

@@ -228,8 +228,9 @@ int fmm3dbie_last_error(void);
 
 /* getnearquad_*_comb_dir(_eval) on device buffers.  kpar6 = (k re, k im, alpha re,
  * alpha im, beta re, beta im); helm = 0 selects the Laplace kernel (real weights).
- * self_n, self_tau_panel: Gauss-Legendre points per direction of the self rule's panels
- * and their largest sinh-parameter length (<= 0: 8 and 1.0).
+ * self_nr, self_nt, self_tau_panel: Gauss-Legendre points of the self rule's panels in
+ * the radial and the sinh-parameter directions, and the panels' largest sinh-parameter
+ * length (<= 0: 8, 10 and 2.0).
  * stats (optional, 24 doubles): nself, nadap, nfixed, overflow, nf, kq; times in ms of
  * prep, fixed, adaptive, self, host tables, selector, plain tree cache, routed tree
  * caches; rule classes used, 0, pairs sent back to the plain integrator, pairs per rule
@@ -249,8 +250,8 @@ int fmm3dbie_nearquad_dev(int helm, int norder, int norder_branch, int64_t npatc
                           const double *srccoefs, const double *srcvals, int64_t ndtarg, int64_t ntarg,
                           const double *targs, const int64_t *ipatch_id, const double *uvs_targ, double eps,
                           const double *kpar6, int64_t nnz, const int64_t *row_ptr, const int64_t *col_ind,
-                          const int64_t *iquad, double rfac0, double *wnear, int self_n, double self_tau_panel,
-                          void *stream, double *stats, int *nref_host, int *class_host, const float *mlp_params,
+                          const int64_t *iquad, double rfac0, double *wnear, int self_nr, int self_nt,
+                          double self_tau_panel, void *stream, double *stats, int *nref_host, int *class_host, const float *mlp_params,
                           float *feat_out, int *pred_host, const int *route_lmax, int *lev_host, int *unc_host);
 
 /* findnearmem / findnear on device arrays (xyzs (3,ns), rads (ns), targets (ndt,nt));

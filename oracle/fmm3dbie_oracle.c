@@ -473,7 +473,7 @@ static void gauss_legendre01(int n, double *x, double *w) {
   }
 }
 
-double orc_self_tau_panel = 1.0; /* max tau-panel length of the self rule */
+double orc_self_tau_panel = 2.0; /* max tau-panel length of the self rule */
 
 /* Self rule used by the GPU path (replaces the unavailable GGQ tables of
  * zget_ggq_self_quad_pt, ggq-quads.f:933-1122): split at the target into 3 triangles

@@ -186,10 +186,17 @@ def self_adap(epsp, norder, uv, srccoefs, targ, kind, zpars, nqorder):
     return out[:, 0] + 1j * out[:, 1]
 
 
-def self_polar(norder, uv, srccoefs, targ, kind, zpars, nr, nt):
+def _set_self_tau(tau):
+    C.c_double.in_dll(lib(), "orc_self_tau_panel").value = tau
+
+
+def self_polar(norder, uv, srccoefs, targ, kind, zpars, nr, nt, tau=2.0):
+    """The self rule at one target: nr x nt Gauss-Legendre points on panels of at most tau
+    in the sinh-graded edge parameter."""
     p = npols(norder)
     out = np.zeros((p, 2))
     targ = _f(targ)
+    _set_self_tau(tau)
     lib().orc_self_polar(norder, p, _f(uv), _f(srccoefs), len(targ), targ, kind,
                          _kp(kind, zpars), nr, nt, out)
     return out[:, 0] + 1j * out[:, 1]
@@ -204,8 +211,10 @@ def centroid_rads(norders, ixyzs, srccoefs):
 
 
 def getnearquad(geom, targs, ipatch_id, uvs_targ, eps, kind, zpars, row_ptr, col_ind, iquad,
-                rfac0, selfmode=SELF_POLAR, self_nr=0, self_nt=0, want_class=False):
-    """geom: dict(norders, ixyzs, iptype, srccoefs (N,9), srcvals (N,12))."""
+                rfac0, selfmode=SELF_POLAR, self_nr=0, self_nt=0, want_class=False, self_tau=2.0):
+    """geom: dict(norders, ixyzs, iptype, srccoefs (N,9), srcvals (N,12)).  The polar self
+    rule uses self_nr x self_nt points on panels of at most self_tau."""
+    _set_self_tau(self_tau)
     norders = np.asarray(geom["norders"], np.int64)
     ixyzs = np.asarray(geom["ixyzs"], np.int64)
     iptype = np.asarray(geom["iptype"], np.int64)

@@ -13,7 +13,7 @@ orders) and are written per pair class."""
 import numpy as np
 import pytest
 
-from fmm3dbie_amd.layer import SELF_N
+from fmm3dbie_amd.layer import SELF_NR, SELF_NT, SELF_TAU
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -53,8 +53,8 @@ def test_laplace_sphere_4x4_o3(cuda):
     P = problem.setup(g, 1e-6, False, pars)
     print("case: laplace sphere 4x4 order 3, %d points, %d pairs" % (len(g["srcvals"]), P["nnz"]), flush=True)
     ref, cls, nref = O.getnearquad(g, g["srcvals"], P["ipatch_id"], P["uvs_targ"], 1e-6, O.LAP_COMB, pars,
-                                   P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_N, SELF_N,
-                                   want_class=True)
+                                   P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_NR, SELF_NT,
+                                   self_tau=SELF_TAU, want_class=True)
     nq = layer.NearQuadDevice(g, False, pars, 1e-6, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], device=cuda)
     w = nq.run(want_detail=True).cpu().numpy()
     assert np.array_equal(nq.stats["cls"], cls)
@@ -104,8 +104,8 @@ def test_wavy_torus_16x6_o6(cuda):
     N = len(g["srcvals"])
     print("case: wavy torus 16x6 order 6 zk 3, %d points, %d pairs" % (N, P["nnz"]), flush=True)
     ref, cls, nref = O.getnearquad(g, g["srcvals"], P["ipatch_id"], P["uvs_targ"], 1e-6, O.HELM_COMB, pars,
-                                   P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_N, SELF_N,
-                                   want_class=True)
+                                   P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_NR, SELF_NT,
+                                   self_tau=SELF_TAU, want_class=True)
     nq = layer.NearQuadDevice(g, True, pars, 1e-6, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], device=cuda)
     w = nq.run(want_detail=True).cpu().numpy()
     assert np.array_equal(nq.stats["cls"], cls)
@@ -159,8 +159,8 @@ def test_wavy_torus_48x16_o6_sampled_rows(cuda):
     iq = np.ones(len(cols) + 1, np.int64)
     iq[1:] += np.cumsum(np.diff(g["ixyzs"])[cols - 1])
     ref, cls, nref = O.getnearquad(g, g["srcvals"][rows], P["ipatch_id"][rows], P["uvs_targ"][rows], 1e-6,
-                                   O.HELM_COMB, pars, sub_ptr, cols, iq, P["rfac0"], O.SELF_POLAR, SELF_N, SELF_N,
-                                   want_class=True)
+                                   O.HELM_COMB, pars, sub_ptr, cols, iq, P["rfac0"], O.SELF_POLAR, SELF_NR, SELF_NT,
+                                   self_tau=SELF_TAU, want_class=True)
     ent = np.concatenate([np.arange(rp[r] - 1, rp[r + 1] - 1) for r in rows])
     assert np.array_equal(st["cls"][ent], cls)
     assert np.array_equal(st["nref"][ent][cls == 1], nref[cls == 1])

@@ -6,7 +6,7 @@ path."""
 import numpy as np
 import pytest
 
-from fmm3dbie_amd.layer import SELF_N
+from fmm3dbie_amd.layer import SELF_NR, SELF_NT, SELF_TAU
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -35,7 +35,7 @@ def _oracle_eval(g, helm, pars, targ, sigma):
     so, wo, ix = near.oversample_geom(g, nov)
     pid = -np.ones(len(targ), np.int64)
     w = O.getnearquad(g, targ, pid, np.zeros((len(targ), 2)), 1e-6, O.HELM_COMB if helm else O.LAP_COMB, pars, rp,
-                      ci, iq, rfac0, O.SELF_POLAR, SELF_N, SELF_N)
+                      ci, iq, rfac0, O.SELF_POLAR, SELF_NR, SELF_NT, self_tau=SELF_TAU)
     return O.lpcomp_addsub(int(helm), g, targ, pars, rp, ci, iq, w, nov, ix, so, wo, sigma)
 
 

@@ -5,7 +5,7 @@ size-independent properties of the apply (linearity, FP32 against FP64)."""
 import numpy as np
 import pytest
 
-from fmm3dbie_amd.layer import SELF_N
+from fmm3dbie_amd.layer import SELF_NR, SELF_NT, SELF_TAU
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -37,8 +37,8 @@ def test_large_nearquad_sampled_rows(cuda, big):
     iq = np.ones(len(cols) + 1, np.int64)
     iq[1:] += np.cumsum(np.diff(g["ixyzs"])[cols - 1])
     ref, cls, nref = O.getnearquad(g, g["srcvals"][rows], P["ipatch_id"][rows], P["uvs_targ"][rows], 1e-6,
-                                   O.HELM_COMB, pars, sub_ptr, cols, iq, P["rfac0"], O.SELF_POLAR, SELF_N, SELF_N,
-                                   want_class=True)
+                                   O.HELM_COMB, pars, sub_ptr, cols, iq, P["rfac0"], O.SELF_POLAR, SELF_NR, SELF_NT,
+                                   self_tau=SELF_TAU, want_class=True)
     ent = np.concatenate([np.arange(rp[r] - 1, rp[r + 1] - 1) for r in rows])
     assert np.array_equal(st["cls"][ent], cls)
     assert np.array_equal(st["nref"][ent][cls == 1], nref[cls == 1])

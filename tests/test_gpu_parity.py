@@ -3,7 +3,7 @@ inputs, plus closed forms and solver-level checks.  Marked gpu."""
 import numpy as np
 import pytest
 
-from fmm3dbie_amd.layer import SELF_N
+from fmm3dbie_amd.layer import SELF_NR, SELF_NT, SELF_TAU
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -35,7 +35,7 @@ def test_nearquad_device_matches_oracle(cuda, kind, norder, na):
     g, helm, pars, P = _case(kind, norder, na)
     ref, cls, nref = O.getnearquad(g, g["srcvals"], P["ipatch_id"], P["uvs_targ"], 1e-6,
                                    O.HELM_COMB if helm else O.LAP_COMB, pars, P["row_ptr"], P["col_ind"], P["iquad"],
-                                   P["rfac0"], O.SELF_POLAR, SELF_N, SELF_N, want_class=True)
+                                   P["rfac0"], O.SELF_POLAR, SELF_NR, SELF_NT, self_tau=SELF_TAU, want_class=True)
     nq = layer.NearQuadDevice(g, helm, pars, 1e-6, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], device=cuda)
     w = nq.run(want_detail=True).cpu().numpy()
     # same pair classification and the same refinement tree (levels 0..20, 1->4)
@@ -77,7 +77,8 @@ def test_eval_offsurface_targets_matches_oracle(cuda):
     iq = near.get_iquad_rsc(g, rp, ci)
     pid = -np.ones(len(targs), np.int64)
     uvs = np.zeros((len(targs), 2))
-    ref = O.getnearquad(g, targs, pid, uvs, 1e-6, O.HELM_COMB, pars, rp, ci, iq, P["rfac0"], O.SELF_POLAR, SELF_N, SELF_N)
+    ref = O.getnearquad(g, targs, pid, uvs, 1e-6, O.HELM_COMB, pars, rp, ci, iq, P["rfac0"], O.SELF_POLAR,
+                        SELF_NR, SELF_NT, self_tau=SELF_TAU)
     w = layer.getnearquad_helm_comb_dir_eval(len(g["norders"]), g["norders"], g["ixyzs"], g["iptype"],
                                              len(g["srcvals"]), g["srccoefs"], g["srcvals"], 3, len(targs), targs,
                                              pid, uvs, 1e-6, pars, 1, len(ci), rp, ci, iq, P["rfac0"], int(iq[-1]) - 1)
@@ -90,7 +91,8 @@ def test_lpcomp_matches_oracle(cuda, kind, norder, na):
     from fmm3dbie_amd import layer
     g, helm, pars, P = _case(kind, norder, na)
     w = O.getnearquad(g, g["srcvals"], P["ipatch_id"], P["uvs_targ"], 1e-6, O.HELM_COMB if helm else O.LAP_COMB,
-                      pars, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_N, SELF_N)
+                      pars, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_NR, SELF_NT,
+                      self_tau=SELF_TAU)
     N = len(g["srcvals"])
     rng = np.random.default_rng(0)
     sig = rng.standard_normal(N) + (1j * rng.standard_normal(N) if helm else 0)
@@ -124,7 +126,7 @@ def test_gmres_matches_oracle_and_point_source(cuda):
     from fmm3dbie_amd import layer, problem
     g, helm, pars, P = _case("helm", 4, 3)
     w = O.getnearquad(g, g["srcvals"], P["ipatch_id"], P["uvs_targ"], 1e-6, O.HELM_COMB, pars, P["row_ptr"],
-                      P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_N, SELF_N)
+                      P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_NR, SELF_NT, self_tau=SELF_TAU)
     zk = 1.1
     x0 = np.array([0.1, 0.2, -0.3])
     rhs = problem.point_source(g["srcvals"][:, :3], x0, zk)
@@ -161,7 +163,7 @@ def test_solver_guru_dropin(cuda):
     from fmm3dbie_amd import layer, problem
     g, helm, pars, P = _case("lap", 3, 2)
     w = O.getnearquad(g, g["srcvals"], P["ipatch_id"], P["uvs_targ"], 1e-6, O.LAP_COMB, pars, P["row_ptr"],
-                      P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_N, SELF_N)
+                      P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_NR, SELF_NT, self_tau=SELF_TAU)
     rhs = problem.point_source(g["srcvals"][:, :3], [0.1, 0.2, -0.3])
     niter, errs, rres, soln = layer.lap_comb_dir_solver_guru(
         len(g["norders"]), g["norders"], g["ixyzs"], g["iptype"], len(g["srcvals"]), g["srccoefs"], g["srcvals"],

@@ -7,7 +7,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from fmm3dbie_amd.layer import SELF_N
+from fmm3dbie_amd.layer import SELF_NR, SELF_NT, SELF_TAU
 from oracle import oracle as O
 
 GOLD = Path(__file__).resolve().parent / "golden"
@@ -168,7 +168,7 @@ def test_self_polar_matches_independent_duffy():
                     k = O.kernel(O.HELM_COMB, pars, np.concatenate([s, nrm / J]), sv[i, :3])
                     tot += wr * wt * r * det * J * k * P
         want = O.rv_umat(n).T @ tot
-        got = O.self_polar(n, u0, sc, sv[i], O.HELM_COMB, pars, SELF_N, SELF_N)
+        got = O.self_polar(n, u0, sc, sv[i], O.HELM_COMB, pars, SELF_NR, SELF_NT, SELF_TAU)
         assert np.abs(got - want).max() < 1e-9 * np.abs(want).max()
 
 
@@ -296,7 +296,7 @@ def test_self_rule_is_the_conformal_disk_principal_value(cfg):
                                   ("slp", O.HELM_SLP, np.array([zk, 0, 0])),
                                   ("dlp", O.HELM_DLP, np.array([zk, 0, 0]))):
             want = _pv_disk_bruteforce(n, uv[i], sc, kind, pars)
-            got = O.self_polar(n, uv[i], sc, sv[i], okind, pars, SELF_N, SELF_N)
+            got = O.self_polar(n, uv[i], sc, sv[i], okind, pars, SELF_NR, SELF_NT, SELF_TAU)
             scale = np.abs(want).max() if scale is None else scale
             assert np.abs(got - want).max() < 1e-7 * scale, (cfg, i, kind)
 
@@ -311,7 +311,7 @@ def test_self_polar_vs_reference_adaptive_self_on_consistent_geometry():
                        (O.HELM_COMB, np.array([2.0, 2j, 1.0]))]:
         for i in [0, 5, 11]:
             a = O.self_adap(1e-13, n, uv[i], sc, sv[i], kind, pars, 16)
-            b = O.self_polar(n, uv[i], sc, sv[i], kind, pars, SELF_N, SELF_N)
+            b = O.self_polar(n, uv[i], sc, sv[i], kind, pars, SELF_NR, SELF_NT, SELF_TAU)
             # the adaptive rule stops refining at level 20 (triaintrouts.f:853), which
             # bounds its own accuracy at the target vertex to ~1e-5 for the DLP
             assert np.abs(a - b).max() < 5e-5 * np.abs(b).max()
@@ -338,7 +338,8 @@ def test_operator_on_sphere_eigenfunction(helm):
     pars = np.array([zk, 1j * zk, 1.0]) if helm else np.array([1.0, 1.0])
     P = problem.setup(g, 1e-7, helm, pars)
     w = O.getnearquad(g, g["srcvals"], P["ipatch_id"], P["uvs_targ"], 1e-7, O.HELM_COMB if helm else O.LAP_COMB,
-                      pars, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_N, SELF_N)
+                      pars, P["row_ptr"], P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_NR, SELF_NT,
+                      self_tau=SELF_TAU)
     sig = g["srcvals"][:, 2].astype(complex if helm else float)
     pot = O.lpcomp_addsub(int(helm), g, g["srcvals"], pars, P["row_ptr"], P["col_ind"], P["iquad"], w, P["novers"],
                           P["ixyzso"], P["srcover"], P["whtsover"], sig)
@@ -406,7 +407,7 @@ def test_oracle_gmres_small_dense():
     pars = np.array([1.0, 1.0])
     P = problem.setup(g, 1e-6, False, pars)
     w = O.getnearquad(g, g["srcvals"], P["ipatch_id"], P["uvs_targ"], 1e-6, O.LAP_COMB, pars, P["row_ptr"],
-                      P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_N, SELF_N)
+                      P["col_ind"], P["iquad"], P["rfac0"], O.SELF_POLAR, SELF_NR, SELF_NT, self_tau=SELF_TAU)
     N = len(g["srcvals"])
     A = np.zeros((N, N))
     for j in range(N):
