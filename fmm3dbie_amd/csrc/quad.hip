@@ -11,12 +11,13 @@
 //                       patch is integrated against it, 16 targets per wavefront with
 //                       the moments on FP64 MFMA (ctriaints_wnodes, triaintrouts.f:67-271
 //                       + umat^T, ggq-quads.f:412).
-//   4. k_tree_cache   : node geometry of the refinement tree's depths 0..3 per patch.
+//   4. k_tree_cache_mfma : node geometry of the refinement tree's depths 0..3 or 0..4
+//                       per patch, 16 patches at a time on FP64 MFMA.
 //   5. k_adaptive     : persistent wavefronts pull (target, patch) pairs from a queue
 //                       and run the 1->4 refinement of triaadap_main
 //                       (triaintrouts.f:1269-1464) depth-first with a per-wave stack in
-//                       HBM; the four children of a step are evaluated together, 64
-//                       nodes at a time.  With learned order selection each pair runs
+//                       HBM; the four children of a step are evaluated together, up
+//                       to 64 nodes at a time (55 at order 6 and above).  With learned order selection each pair runs
 //                       the XG rule of its predicted class.
 //   6. k_self         : one wavefront per on-surface target; a polar/sinh product rule
 //                       on the three triangles meeting at the target replaces the GGQ
@@ -525,34 +526,117 @@ __host__ __device__ constexpr int cache_parents(int depth) {
   return depth <= 1 ? 1 : (depth == 2 ? 5 : (depth == 3 ? 21 : NCACHE_PARENT));
 }
 
+// The cache on the matrix cores.  For one patch the geometry of every cached node is
+// the product of the basis table [nodes x NP] with the patch's coefficients [NP x 9]; with
+// 16 patches side by side each of the 9 components is a product [nodes x NP] x [NP x 16]:
+// per 16-node tile, ceil(NP / 4) v_mfma_f64_16x16x4f64 steps per component, A = the
+// tile's basis values (4 functions per step, lane l: node l % 16, function l / 16),
+// B = 4 coefficients of the 16 patches (lane l: function l / 16, patch l % 16), read from
+// the workgroup's LDS copy of the 16 patches' coefficients.  Accumulator element r of lane
+// l is node i0 + l / 16 + 4 r of patch p0 + l % 16, whose 9 components the lane then holds
+// together and turns into the 64-byte node record.  Grid: x = groups of 16 patches, y =
+// slices of the node tiles (4 wavefronts per workgroup, one tile at a time each).  Three
+// wavefronts per SIMD (the compiler's register choice at the default bound leaves two);
+// the matrix-core version takes 4.3 ms on 12288 order-6 patches of 341 triangles of 55
+// nodes, where one thread per node with vector FMAs took 7.5 ms.
 template <int N>
-__global__ void k_tree_cache(int64_t npatches, int kq, int ntri, const double* __restrict__ tuv,
-                             const double* __restrict__ twq, const double* __restrict__ ptab,
-                             const int64_t* __restrict__ ixyzs, const double* __restrict__ srccoefs,
-                             double* __restrict__ cache) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_tree_cache_mfma(int64_t npatches, int ncn, const double* __restrict__ twq, const double* __restrict__ ptab,
+                  const int64_t* __restrict__ ixyzs, const double* __restrict__ srccoefs,
+                  double* __restrict__ cache) {
   constexpr int NP = (N + 1) * (N + 2) / 2;
-  const int64_t ip = blockIdx.x;
-  const int nn = ntri * kq;
-  const double* sc = srccoefs + 9 * (ixyzs[ip] - 1);
-  double* out = cache + (size_t)ip * CACHE_STRIDE * nn;
-  for (int q = threadIdx.x; q < nn; q += blockDim.x) {
-    double g[12];
-    for (int m = 0; m < 9; m++) g[m] = 0.0;
-    for (int l = 0; l < NP; l++) {
-      const double pl = ptab[(size_t)q * ptab_w(NP) + 2 + l];
-      for (int m = 0; m < 9; m++) g[m] = fma(sc[9 * l + m], pl, g[m]);
-    }
-    const double jac = geom_finish(g);
-    double* o = out + (size_t)CACHE_STRIDE * q;
-    o[0] = g[0];
-    o[1] = g[1];
-    o[2] = g[2];
-    o[3] = g[9];
-    o[4] = g[10];
-    o[5] = g[11];
-    o[6] = jac * twq[q];
-    o[7] = 0.0;
+  constexpr int KS = (NP + 3) / 4;
+  constexpr int PW = ptab_w(NP);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ia = lane & 15, ka = lane >> 4;
+  const int64_t p = (int64_t)blockIdx.x * 16 + ia;
+  const bool pok = p < npatches;
+  // B fragments of the 16 patches in LDS: step s, component m, lane l at ((s 9 + m) 64 + l)
+  __shared__ double bs[KS * 9 * 64];
+  __shared__ __align__(16) double rec[4 * 512];  // per wavefront: 64 records of 8 doubles
+  for (int i = threadIdx.x; i < KS * 9 * 64; i += blockDim.x) {
+    const int li = i & 63, sm = i >> 6;
+    const int s = sm / 9, m = sm - 9 * s;
+    const int l = 4 * s + (li >> 4);
+    const int64_t pi = (int64_t)blockIdx.x * 16 + (li & 15);
+    bs[i] = (pi < npatches && l < NP) ? srccoefs[9 * (ixyzs[pi] - 1) + 9 * l + m] : 0.0;
   }
+  __syncthreads();
+  double* out = cache + (size_t)(pok ? p : 0) * CACHE_STRIDE * ncn;
+  const int ntile = (ncn + 15) / 16;
+  const int tstep = 4 * gridDim.y;
+  // A fragments of the next tile are loaded while the current one is multiplied
+  double an[KS];
+  auto load_a = [&](int tt) {
+    // rows 16 tt .. 16 tt + 15 of the table (padded by 64 rows past its last node)
+    const double* arow = ptab + (size_t)(16 * tt + ia) * PW + 2;
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+      const int l = 4 * s + ka;
+      an[s] = l < NP ? arow[l] : 0.0;
+    }
+  };
+  int t = blockIdx.y * 4 + wv;
+  if (t < ntile) load_a(t);
+  for (; t < ntile; t += tstep) {
+    const int i0 = 16 * t;
+    double a[KS];
+#pragma unroll
+    for (int s = 0; s < KS; s++) a[s] = an[s];
+    if (t + tstep < ntile) load_a(t + tstep);
+    // read the B fragments from LDS in every tile rather than keep them in 126 registers
+    asm volatile("" ::: "memory");
+    dbl4 acc[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) acc[m] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+#pragma unroll
+      for (int m = 0; m < 9; m++)
+        acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], bs[(s * 9 + m) * 64 + lane], acc[m], 0, 0, 0);
+    }
+    // records through LDS, 64 at a time (node i0 + ka + 4 r of patch p0 + ia at slot
+    // 4 ia + ka), then stored as 16-byte pieces by consecutive lanes: each store
+    // instruction writes 4 patches' 4 consecutive records (256 contiguous bytes each)
+    double* st = rec + wv * 512;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int n = i0 + ka + 4 * r;
+      double g[12];
+#pragma unroll
+      for (int m = 0; m < 9; m++) g[m] = acc[m][r];
+      const double jac = geom_finish(g);
+      double* o = st + 8 * (4 * ia + ka);
+      *reinterpret_cast<dbl4*>(o) = dbl4{g[0], g[1], g[2], g[9]};
+      *reinterpret_cast<dbl4*>(o + 4) = dbl4{g[10], g[11], jac * (n < ncn ? twq[n] : 0.0), 0.0};
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the wavefront's own LDS writes
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int q = 64 * k + lane, rec_i = q >> 2, part = q & 3;
+        const int64_t pp = (int64_t)blockIdx.x * 16 + (rec_i >> 2);
+        const int nn = i0 + (rec_i & 3) + 4 * r;
+        if (pp < npatches && nn < ncn)
+          *reinterpret_cast<double2*>(cache + ((size_t)pp * ncn + nn) * CACHE_STRIDE + 2 * part) =
+              *reinterpret_cast<const double2*>(st + 2 * q);
+      }
+      __builtin_amdgcn_s_waitcnt(0xC07F);
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
+template <int N>
+void launch_tree_cache(int64_t npat, int ncn, const double* twq, const double* ptab, const int64_t* ixyzs,
+                       const double* srccoefs, double* cache, hipStream_t s) {
+  const int64_t gx = (npat + 15) / 16;
+  const int ntile = (ncn + 15) / 16;
+  // about 4096 workgroups, each wavefront taking every (4 gy)-th tile of its slice
+  int64_t gy = (4096 + gx - 1) / gx;
+  if (gy > (ntile + 3) / 4) gy = (ntile + 3) / 4;
+  if (gy < 1) gy = 1;
+  k_tree_cache_mfma<N><<<dim3((unsigned)gx, (unsigned)gy), 256, 0, s>>>(npat, ncn, twq, ptab, ixyzs, srccoefs,
+                                                                         cache);
 }
 
 // Rules of the adaptive integrator, one per class: the XG rule (kq nodes), the node
@@ -1173,15 +1257,15 @@ T* wsalloc(int slot, size_t n) {
 // Patch-independent tables of one (order, rule) combination, built once per device.
 struct OrderTables {
   int kq = 0, nf = 0;
-  double *umat, *rvw, *rvuv, *xg, *fuv, *Pq, *Bq, *tuv, *twq, *ptab;
+  double *umat, *rvw, *rvuv, *xg, *fuv, *Pq, *Bq, *twq, *ptab;
 };
 std::map<std::tuple<int, int, int, int, int>, OrderTables> g_tables;
 
 // The adaptive integrator's XG rule of degree q with the node tables of the cached tree
-// (depths 0..3) for patches of order N; built once per device.
+// (depths 0..4) for patches of order N; built once per device.
 struct RuleTables {
   int kq = 0;
-  double *xg, *tuv, *twq, *ptab;
+  double *xg, *twq, *ptab;
 };
 std::map<std::tuple<int, int, int>, RuleTables> g_rules;
 
@@ -1325,11 +1409,10 @@ int build_tables(int nqorder, int nlev, int nqorder_f, OrderTables& T, hipStream
   T.fuv = upload(fuv, s);
   T.Pq = upload(Pq, s);
   T.Bq = upload(Bq, s);
-  T.tuv = upload(tuv, s);
   T.twq = upload(twq, s);
   T.ptab = upload(ptab, s);
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  if (!T.umat || !T.rvw || !T.rvuv || !T.xg || !T.fuv || !T.Pq || !T.Bq || !T.tuv || !T.twq || !T.ptab) return -2;
+  if (!T.umat || !T.rvw || !T.rvuv || !T.xg || !T.fuv || !T.Pq || !T.Bq || !T.twq || !T.ptab) return -2;
   return 0;
 }
 
@@ -1349,11 +1432,10 @@ int rule_tables(int q, const RuleTables** out, hipStream_t s) {
     RuleTables T;
     T.kq = kq;
     T.xg = upload(xg, s);
-    T.tuv = upload(tuv, s);
     T.twq = upload(twq, s);
     T.ptab = upload(ptab, s);
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
-    if (!T.xg || !T.tuv || !T.twq || !T.ptab) return -2;
+    if (!T.xg || !T.twq || !T.ptab) return -2;
     it = g_rules.emplace(key, T).first;
   }
   *out = &it->second;
@@ -1479,7 +1561,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   const int npar_plain = cache_parents(A.nnz >= 300 * npat ? 4 : 3);
   const int ntri_plain = cache_tris(npar_plain);
   const int ncn = ntri_plain * kq;
-  double *d_tuv = TB.tuv, *d_twq = TB.twq, *d_ptab = TB.ptab;
+  double *d_twq = TB.twq, *d_ptab = TB.ptab;
   const bool early_tree = s2 != s && A.ipatch_id != nullptr && npat > 0;
   double* d_cache = nullptr;
   const int tb = 256;
@@ -1496,8 +1578,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
     HCHK(hipEventRecord(ev[14], s));
     HCHK(hipStreamWaitEvent(s2, ev[14], 0));
     HCHK(hipEventRecord(ev[9], s2));
-    k_tree_cache<N><<<npat, 256, 0, s2>>>(npat, kq, ntri_plain, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs,
-                                          d_cache);
+    launch_tree_cache<N>(npat, ntri_plain * kq, d_twq, d_ptab, A.ixyzs, A.srccoefs, d_cache, s2);
     HCHK(hipGetLastError());
     HCHK(hipEventRecord(ev[8], s2));
   }
@@ -1632,8 +1713,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
       HCHK(hipStreamWaitEvent(s, ev[8], 0));
     } else {
       HCHK(hipEventRecord(ev[9], s));
-      k_tree_cache<N><<<npat, 256, 0, s>>>(npat, kq, ntri_plain, d_tuv, d_twq, d_ptab, A.ixyzs, A.srccoefs,
-                                           d_cache);
+      launch_tree_cache<N>(npat, ntri_plain * kq, d_twq, d_ptab, A.ixyzs, A.srccoefs, d_cache, s);
       HCHK(hipEventRecord(ev[8], s));
     }
     HCHK(hipEventRecord(ev[11], s));
@@ -1662,8 +1742,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
         const int ntri = cache_tris(R.npar[c]);
         double* cc = wsalloc<double>(W_RCACHE0 + c, (size_t)npat * CACHE_STRIDE * ntri * RT->kq);
         if (!cc) return -2;
-        k_tree_cache<N><<<npat, 256, 0, s>>>(npat, RT->kq, ntri, RT->tuv, RT->twq, RT->ptab, A.ixyzs, A.srccoefs,
-                                             cc);
+        launch_tree_cache<N>(npat, ntri * RT->kq, RT->twq, RT->ptab, A.ixyzs, A.srccoefs, cc, s);
         R.cache[c] = cc;
       }
       HCHK(hipGetLastError());
