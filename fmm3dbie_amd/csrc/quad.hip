@@ -16,8 +16,9 @@
 //   5. k_adaptive     : persistent wavefronts pull (target, patch) pairs from a queue
 //                       and run the 1->4 refinement of triaadap_main
 //                       (triaintrouts.f:1269-1464) depth-first with a per-wave stack in
-//                       HBM; the four children of a step are evaluated together, up
-//                       to 64 nodes at a time (55 at order 6 and above).  With learned order selection each pair runs
+//                       HBM; the nodes of a step's four children are evaluated up
+//                       to 64 at a time (55 at order 6 and above, where a rule of 28
+//                       to 55 nodes takes one chunk per child).  With learned order selection each pair runs
 //                       the XG rule of its predicted class.
 //   6. k_self         : one wavefront per on-surface target; a polar/sinh product rule
 //                       on the three triangles meeting at the target replaces the GGQ
@@ -774,6 +775,100 @@ __device__ __forceinline__ void step_moments(double* pb, const double* sc, bool 
   }
 }
 
+// Column sums of rows 0 <= q < nr (the rows of one triangle): the G = 64 / NP lane groups
+// take rows G i + grp for i < nr / G (a wave-uniform trip count), then groups
+// grp < nr mod G one more row each.
+template <int NP, bool HELM>
+__device__ __forceinline__ Mom column_sum_tri(const double* pb, int nr) {
+  constexpr int PW = ptab_w(NP);
+  constexpr int G = WAVE / NP;
+  const int lane = threadIdx.x;
+  const int grp = lane / NP, j = lane - grp * NP;
+  double sr = 0.0, si = 0.0;
+  if (grp < G) {
+    const double* b = pb + grp * PW + 2 + j;
+    const int nfull = nr / G;
+#pragma unroll 8
+    for (int i = 0; i < nfull; i++) {
+      const double p = b[G * i * PW];
+      const double2 f = *reinterpret_cast<const double2*>(b - 2 - j + G * i * PW);
+      sr = fma(f.x, p, sr);
+      if (HELM) si = fma(f.y, p, si);
+    }
+    if (grp < nr - G * nfull) {
+      const double p = b[G * nfull * PW];
+      const double2 f = *reinterpret_cast<const double2*>(b - 2 - j + G * nfull * PW);
+      sr = fma(f.x, p, sr);
+      if (HELM) si = fma(f.y, p, si);
+    }
+  }
+  const double pr = sr, pi = si;
+#pragma unroll
+  for (int k = 1; k < G; k++) {
+    sr += __shfl(pr, (lane + k * NP) & 63, 64);
+    if (HELM) si += __shfl(pi, (lane + k * NP) & 63, 64);
+  }
+  return lane < NP ? Mom{sr, HELM ? si : 0.0} : Mom{0.0, 0.0};
+}
+
+// step_moments for rules of at most chunk_rows(N) nodes: one chunk per triangle, so the
+// triangle index is wave-uniform and the column sums run a wave-uniform trip count (on the
+// deformed sphere of order 6 with the 55-node rule the adaptive kernel took 75.8 ms through
+// step_moments, with the same chunks but per-lane child indices and per-lane row bounds in
+// the column sums, and 71.0 ms here).
+template <int N, bool HELM, bool CACHED>
+__device__ __forceinline__ void step_tris(double* pb, const double* sc, bool root, int tid, double x0, double y0,
+                                          double dx, double dy, int kq, const double* __restrict__ xg,
+                                          const double* __restrict__ cpatch, const double* __restrict__ ptab,
+                                          const double* tg, const KPar& kp, Mom (&m)[4]) {
+  constexpr int NP = (N + 1) * (N + 2) / 2;
+  constexpr int PW = ptab_w(NP);
+  const int lane = threadIdx.x;
+  const int ntri = root ? 1 : 4;
+  const int t0 = root ? 0 : cached_child(tid, 0);
+  const double hx = 0.5 * dx, hy = 0.5 * dy, da = hx * hx;
+#pragma unroll
+  for (int c = 0; c < 4; c++) m[c] = Mom{0.0, 0.0};
+  for (int c = 0; c < ntri; c++) {
+    if (CACHED) {
+      const double* blk = ptab + ((size_t)(t0 + c) * kq) * PW;
+#pragma unroll
+      for (int k = 0; k < chunk_blocks(N); k++)
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void*)(blk + 2 * (k * WAVE + lane)),
+            (__attribute__((address_space(3))) void*)(pb + 2 * k * WAVE), 16, 0, 0);
+    }
+    double2 fv = make_double2(0.0, 0.0);
+    if (lane < kq) {
+      double g[12], w;
+      if (CACHED) {
+        const double* cn = cpatch + (size_t)CACHE_STRIDE * ((size_t)(t0 + c) * kq + lane);
+        g[0] = cn[0];
+        g[1] = cn[1];
+        g[2] = cn[2];
+        g[9] = cn[3];
+        g[10] = cn[4];
+        g[11] = cn[5];
+        w = cn[6];
+      } else {
+        const double cx = (c & 1) ? x0 + hx : x0, cy = (c & 2) ? y0 + hy : y0;
+        const double cdx = c == 3 ? -hx : hx, cdy = c == 3 ? -hy : hy;
+        const double u = cx + cdx * xg[3 * lane], v = cy + cdy * xg[3 * lane + 1];
+        koorn_geom_row<N>(u, v, sc, g, pb + lane * PW + 2);
+        w = geom_finish(g) * da * xg[3 * lane + 2];
+      }
+      double vr, vi;
+      kern_eval<HELM>(tg, g, kp, vr, vi);
+      fv = make_double2(vr * w, vi * w);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    if (lane < kq) *reinterpret_cast<double2*>(pb + lane * PW) = fv;
+    __syncthreads();
+    m[c] = column_sum_tri<NP, HELM>(pb, kq);
+    __syncthreads();
+  }
+}
+
 // Adaptive integration (triaadap_main, triaintrouts.f:1269-1464): persistent wavefronts
 // pull (target, patch) pairs from a queue; each pair's refinement tree is walked depth
 // first from a per-wavefront stack (the leaf set is that of the reference's level-by-level
@@ -846,7 +941,12 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
     const int npar = R.npar[cls];
     const double* cpatch = R.cache[cls] + (size_t)ip * CACHE_STRIDE * cache_tris(npar) * kq;
     Mom mc[4];
-    step_moments<N, HELM, true>(pb, lsc, true, npar, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
+    // (at order 6 and above: at lower orders the second path costs a wavefront per SIMD)
+    const bool per_tri = N >= 6 && kq <= chunk_rows(N) && 2 * kq > chunk_rows(N);
+    if (per_tri)
+      step_tris<N, HELM, true>(pb, lsc, true, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
+    else
+      step_moments<N, HELM, true>(pb, lsc, true, npar, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
     if (lane < 6) stack[lane] = lane < 2 ? 0.0 : (lane < 4 ? 1.0 : 0.0);  // level 0, tree id 0
     if (lane < NP) {
       stack[6 + lane] = mc[0].re;
@@ -863,10 +963,16 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
       const int level = __builtin_amdgcn_readfirstlane((int)e[4]);
       const int tid = __builtin_amdgcn_readfirstlane((int)e[5]);  // cached tree id, -1 below depth 2
       const Mom mpar = lane < NP ? Mom{e[6 + lane], e[6 + NP + lane]} : Mom{0.0, 0.0};
-      if (tid >= 0 && tid < npar)
+      if (per_tri) {
+        if (tid >= 0 && tid < npar)
+          step_tris<N, HELM, true>(pb, lsc, false, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
+        else
+          step_tris<N, HELM, false>(pb, lsc, false, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
+      } else if (tid >= 0 && tid < npar) {
         step_moments<N, HELM, true>(pb, lsc, false, npar, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
-      else
+      } else {
         step_moments<N, HELM, false>(pb, lsc, false, npar, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
+      }
       nref++;
       nunc += (tid < 0 || tid >= npar) ? 1 : 0;  // children below the cached tree
       Mom ssum = mc[0];

@@ -90,12 +90,22 @@ def chunk_rows(norder):
     return 55 if norder >= 6 else 64
 
 
+def chunks(norder, kq):
+    """Chunks of the adaptive kernel's root step and of a refinement test with a rule of
+    kq nodes: one per triangle at order 6 and above when a triangle fills more than half
+    a chunk (quad.hip step_tris), else the nodes of the step in equal chunks."""
+    cr = chunk_rows(norder)
+    if norder >= 6 and cr < 2 * kq <= 2 * cr:
+        return 1, 4
+    return -(-kq // cr), -(-4 * kq // cr)
+
+
 def collect(nq, stream=None):
     """Features of every adaptive pair (from a plain run) and its label: the rule class
     whose adaptive integration takes the least time.  Every pair is integrated once
     with each class (forced routing, no level cap, every kernel in order on one stream).
     The kernel works in chunks of at most chunk_rows(norder) nodes (one node per lane;
-    the root is ceil(kq/c) chunks, a refinement test's four children ceil(4kq/c)), and a
+    chunks() counts them for the root and a refinement test), and a
     chunk costs the same whether its lanes are all busy or not, so the measured time of
     each class's launch is fitted by a cost per chunk on the cached tree, a
     cost per chunk below it (geometry evaluated) and a cost per pair, summed over the
@@ -136,8 +146,7 @@ def collect(nq, stream=None):
             unc = st["unc"][idx].astype(np.float64)
             ncached[c] = kq[c] * (1 + 4 * (nref - unc))
             ncomp[c] = kq[c] * 4 * unc
-            cr = chunk_rows(nq.norder)
-            croot, cstep = -(-kq[c] // cr), -(-4 * kq[c] // cr)
+            croot, cstep = chunks(nq.norder, kq[c])
             ccached[c] = croot + cstep * (nref - unc)
             ccomp[c] = cstep * unc
             lev[c] = st["lev"][idx]

@@ -111,3 +111,17 @@ def test_open_classes_pays_for_each_tree_cache():
     # the plain class's cache is free; classes of the plain degree too
     c = S.tree_cost(0.3, [33, 42, 55, 79, 112, 150], 16)
     assert c[P] == 0.0 and abs(c[5] - 0.3 * 150 / 55) < 1e-15
+
+
+def test_chunk_counts_follow_the_kernels_chunking():
+    """Chunks of the adaptive kernel's root step and refinement test (the cost model's
+    unit): one per triangle at order 6 for rules of 28..55 nodes, else equal chunks of at
+    most 55 (order >= 6) or 64 nodes."""
+    from fmm3dbie_amd import selector as S
+    assert S.chunks(6, 55) == (1, 4)
+    assert S.chunks(6, 33) == (1, 4)
+    assert S.chunks(6, 16) == (1, 2)
+    assert S.chunks(6, 79) == (2, 6)
+    assert S.chunks(6, 171) == (4, 13)
+    assert S.chunks(4, 55) == (1, 4)
+    assert S.chunks(4, 79) == (2, 5)
