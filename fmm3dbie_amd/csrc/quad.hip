@@ -1843,8 +1843,11 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
           continue;
         }
         // the class's pairs evaluate no triangle below depth lmax (deeper ones go back to
-        // the plain integrator), so its cache holds depths 0 .. min(3, lmax)
-        R.npar[c] = cache_parents(R.lmax[c] < 3 ? R.lmax[c] : 3);
+        // the plain integrator); its cache holds depths 0 .. min(2, lmax): the pairs routed
+        // to a rule of another degree mostly end at levels 1 and 2 (on the wavy torus, 99%
+        // of them), and a deeper cache would cost every generation 4 times as much for
+        // the few refinement tests below it
+        R.npar[c] = cache_parents(R.lmax[c] < 2 ? R.lmax[c] : 2);
         const int ntri = cache_tris(R.npar[c]);
         double* cc = wsalloc<double>(W_RCACHE0 + c, (size_t)npat * CACHE_STRIDE * ntri * RT->kq);
         if (!cc) return -2;

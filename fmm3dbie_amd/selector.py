@@ -90,13 +90,20 @@ def chunk_rows(norder):
     return 55 if norder >= 6 else 64
 
 
+def per_triangle(norder, kq):
+    """Whether the adaptive kernel takes one chunk per triangle for a rule of kq nodes
+    (order 6 and above, a triangle filling more than half a chunk: quad.hip step_tris)."""
+    cr = chunk_rows(norder)
+    return norder >= 6 and cr < 2 * kq <= 2 * cr
+
+
 def chunks(norder, kq):
     """Chunks of the adaptive kernel's root step and of a refinement test with a rule of
-    kq nodes: one per triangle at order 6 and above when a triangle fills more than half
-    a chunk (quad.hip step_tris), else the nodes of the step in equal chunks."""
-    cr = chunk_rows(norder)
-    if norder >= 6 and cr < 2 * kq <= 2 * cr:
+    kq nodes: one per triangle (per_triangle), else the nodes of the step in equal
+    chunks."""
+    if per_triangle(norder, kq):
         return 1, 4
+    cr = chunk_rows(norder)
     return -(-kq // cr), -(-4 * kq // cr)
 
 
@@ -107,15 +114,17 @@ def collect(nq, stream=None):
     The kernel works in chunks of at most chunk_rows(norder) nodes (one node per lane;
     chunks() counts them for the root and a refinement test), and a
     chunk costs the same whether its lanes are all busy or not, so the measured time of
-    each class's launch is fitted by a cost per chunk on the cached tree, a
+    each class's launch is fitted by a cost per chunk on the cached tree (one for the
+    classes with a chunk per triangle, one for the others: the two code paths), a
     cost per chunk below it (geometry evaluated) and a cost per pair, summed over the
     pairs' own counts; the label is the class of least fitted chunk cost among the
     classes worth their tree cache (open_classes).  Returns (X
     (nadap, 8) float32, y (nadap,) int32, CSR entry index of each adaptive pair, info
     with the per-class fitted cost 'cost', kernel evaluations 'evals', deepest levels
     'lev' (6, nadap), launch times 'ms', the fitted unit costs 'unit' (ms per cached
-    chunk, per computed chunk, per pair), node counts 'kq', the plain degree 'q0', the
-    classes used 'used' and the plain rule's tree-cache time for depths 0..3 'ms_tree')."""
+    chunk of the per-triangle classes and of the others, per computed chunk, per pair),
+    node counts 'kq', the plain degree 'q0', the
+    classes used 'used' and the plain rule's tree-cache time for depths 0..2 'ms_tree')."""
     from scipy.optimize import nnls
     from . import layer
     torch = _torch()
@@ -126,7 +135,7 @@ def collect(nq, stream=None):
     try:
         nq.run(stream=stream, want_detail=True, features=feat)
         torch.cuda.synchronize()
-        ms_tree = nq.stats["ms_tree"] * 85.0 / max(nq.stats["cache_tris"], 1)  # per depth-0..3 cache
+        ms_tree = nq.stats["ms_tree"] * 21.0 / max(nq.stats["cache_tris"], 1)  # per depth-0..2 cache
         idx = np.nonzero(nq.stats["cls"] == 1)[0]
         X = feat.cpu().numpy()[idx]
         q0 = plain_degree(nq.stats["kq"])
@@ -153,20 +162,22 @@ def collect(nq, stream=None):
             ms[c] = st["ms_adap"]
     finally:
         layer.set_side_stream(side)
-    # a cost per pair (queue, staging, weights: the same for every class) keeps the two
-    # chunk costs from absorbing it
-    A = np.stack([ccached.sum(1), ccomp.sum(1), np.full(NCLASS, float(len(idx)))], 1)
+    # a cost per pair (queue, staging, weights: the same for every class) keeps the chunk
+    # costs from absorbing it
+    tri = np.array([per_triangle(nq.norder, int(k)) for k in kq])
+    A = np.stack([np.where(tri, ccached.sum(1), 0.0), np.where(tri, 0.0, ccached.sum(1)), ccomp.sum(1),
+                  np.full(NCLASS, float(len(idx)))], 1)
     unit, _ = nnls(A, ms)
-    cost = unit[0] * ccached + unit[1] * ccomp
+    cost = np.where(tri, unit[0], unit[1])[:, None] * ccached + unit[2] * ccomp
     y, used = open_classes(cost, tree_cost(ms_tree, kq, q0))
     return X, y, idx, dict(cost=cost, evals=ncached + ncomp, lev=lev, ms=ms, unit=unit, kq=kq, q0=q0,
                            used=used, ms_tree=ms_tree)
 
 
 def tree_cost(ms_tree, kq, q0):
-    """Time of the tree cache (depths 0..3) of each rule class, scaled from the plain
-    rule's (ms_tree, for depths 0..3) by the node count; 0 for the plain rule, whose cache
-    every generation builds."""
+    """Time of the tree cache of each routed rule class (depths 0..2, quad.hip), scaled
+    from the plain rule's (ms_tree, for depths 0..2) by the node count; 0 for the plain
+    rule, whose cache every generation builds."""
     kq = np.asarray(kq, np.float64)
     c = ms_tree * kq / kq[PLAIN_CLASS]
     c[[class_degree(q0, k) == q0 for k in range(NCLASS)]] = 0.0

@@ -237,7 +237,7 @@ int fmm3dbie_last_error(void);
  * class (6), triangles of the plain rule's tree cache (85: depths 0..3; 341: depths 0..4,
  * used from 300 near pairs per patch).  nref_host/class_host/lev_host/unc_host (optional, nnz ints): refinement
  * tests per pair, pair class (0 self, 1 adaptive, 2 fixed), deepest level of the pair's
- * tree, refinement tests below the cached depths 0..3 (children geometry computed).
+ * tree, refinement tests below the cached depths (children geometry computed).
  * mlp_params (optional, device, fmm3dbie_mlp_nparam() floats): learned order selection.
  * Every adaptive pair (ggq-quads.f:382-386) gets a rule class c = 0..5 from the network:
  * the adaptive integrator of triaintrouts.f:1269-1464 with the XG rule of degree
