@@ -421,44 +421,6 @@ struct Mom {
   }
 };
 
-// Moments of a set of staged nodes, M_j = sum_q f_q P_j(q): row q of buf holds
-// (f_re, f_im, P_0..P_{NP-1}) of node q (written by the lane that evaluated it).  The
-// G = 64 / NP lane groups split the rows (lane g NP + j takes column j of rows g, g + G,
-// ...), one basis read serving the real and imaginary parts; lane j < NP adds the other
-// groups' partial sums and returns M_j.
-template <int NP, bool HELM>
-__device__ __forceinline__ Mom column_sum(const double* buf, int nrows) {
-  static_assert(NP <= WAVE, "one moment component per lane");
-  constexpr int W = row_w(NP);
-  constexpr int G = WAVE / NP;
-  const int lane = threadIdx.x;
-  __syncthreads();
-  const int grp = lane / NP, j = lane - grp * NP;
-  double sr = 0.0, si = 0.0;
-  if (grp < G) {
-#pragma unroll 8
-    for (int q = grp; q < nrows; q += G) {
-      const double* row = buf + q * W;
-      const double p = row[2 + j];
-      if (HELM) {
-        const double2 f = *reinterpret_cast<const double2*>(row);
-        sr = fma(f.x, p, sr);
-        si = fma(f.y, p, si);
-      } else {
-        sr = fma(row[0], p, sr);
-      }
-    }
-  }
-  const double pr = sr, pi = si;  // shuffle the unmodified partial sums
-#pragma unroll
-  for (int k = 1; k < G; k++) {
-    sr += __shfl(pr, (lane + k * NP) & 63, 64);
-    if (HELM) si += __shfl(pi, (lane + k * NP) & 63, 64);
-  }
-  __syncthreads();
-  return lane < NP ? Mom{sr, HELM ? si : 0.0} : Mom{0.0, 0.0};
-}
-
 // Apply umat^T to the moments (lane j < NP holds M_j) and store the weights of one pair;
 // output c is weight c/2, part c%2 (complex) or weight c (real), so the stores coalesce.
 template <int NP, bool HELM>
@@ -775,12 +737,13 @@ __device__ __forceinline__ void step_moments(double* pb, const double* sc, bool 
   }
 }
 
-// Column sums of rows 0 <= q < nr (the rows of one triangle): the G = 64 / NP lane groups
+// Moments of staged rows 0 <= q < nr, M_j = sum_q f_q P_j(q) (row q at pb + q PW holds
+// f_q (re, im), then P_0..P_{NP-1}; lane j < NP returns M_j): the G = 64 / NP lane groups
 // take rows G i + grp for i < nr / G (a wave-uniform trip count), then groups
-// grp < nr mod G one more row each.
-template <int NP, bool HELM>
+// grp < nr mod G one more row each, one basis read serving the real and imaginary parts;
+// the groups' partial sums are added with shuffles.
+template <int NP, bool HELM, int PW = ptab_w(NP)>
 __device__ __forceinline__ Mom column_sum_tri(const double* pb, int nr) {
-  constexpr int PW = ptab_w(NP);
   constexpr int G = WAVE / NP;
   const int lane = threadIdx.x;
   const int grp = lane / NP, j = lane - grp * NP;
@@ -1275,7 +1238,9 @@ __global__ void __launch_bounds__(64) k_self(int nself, const int2* __restrict__
       row[1] = vi * w;
     }
     const int nrows = total - base < WAVE ? total - base : WAVE;
-    acc += column_sum<NP, HELM>(buf, nrows);
+    __syncthreads();
+    acc += column_sum_tri<NP, HELM, row_w(NP)>(buf, nrows);
+    __syncthreads();
   }
   if (has_dl) {
     // the disk correction of every ray (tau node): rays spread over the lanes
