@@ -836,7 +836,8 @@ __device__ __forceinline__ void step_tris(double* pb, const double* sc, bool roo
 // pull (target, patch) pairs from a queue; each pair's refinement tree is walked depth
 // first from a per-wavefront stack (the leaf set is that of the reference's level-by-level
 // walk: every decision depends only on its own triangle).  A step evaluates the four
-// children of the popped triangle together and refines iff max_j |sum_c M_c,j - M_j| >
+// children of the popped triangle (step_tris: one chunk per child; step_moments: the
+// children's nodes in equal chunks) and refines iff max_j |sum_c M_c,j - M_j| >
 // epsp (triaintrouts.f:1435-1448); the children are pushed only when refined.  A pair of
 // a routed class whose tree needs leaves below the class's level cap (lmax < NLMAX) is
 // appended to fb_out
