@@ -911,22 +911,34 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
       step_tris<N, HELM, true>(pb, lsc, true, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
     else
       step_moments<N, HELM, true>(pb, lsc, true, npar, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
-    if (lane < 6) stack[lane] = lane < 2 ? 0.0 : (lane < 4 ? 1.0 : 0.0);  // level 0, tree id 0
-    if (lane < NP) {
-      stack[6 + lane] = mc[0].re;
-      stack[6 + NP + lane] = mc[0].im;
-    }
-    int sp = 1, nref = 0, deep = 0, nunc = 0;
-    bool aborted = false;
+    // The triangle being tested is held in registers and the stack holds the others:
+    // a refinement pushes children 0..2 and goes on with child 3 (the one the stack
+    // order takes next).  At order 6 and above the entry on top of the stack is loaded at
+    // the start of each test, so a leaf's successor is in registers when the test ends
+    // (at lower orders those registers would cost a wavefront per SIMD).
+    constexpr bool PREFETCH = N >= 6;
+    double cx0 = 0.0, cy0 = 0.0, cdx = 1.0, cdy = 1.0;
+    int clevel = 0, ctid = 0;
+    Mom cpar = lane < NP ? mc[0] : Mom{0.0, 0.0};
+    int sp = 0, nref = 0, deep = 0, nunc = 0;
+    bool aborted = false, have = true;
     Mom result{0.0, 0.0};
-    while (sp > 0) {
-      sp--;
-      const double* e = stack + (size_t)sp * ES;
-      const double x0 = wave_uniform(e[0]), y0 = wave_uniform(e[1]), dx = wave_uniform(e[2]),
-                   dy = wave_uniform(e[3]);
-      const int level = __builtin_amdgcn_readfirstlane((int)e[4]);
-      const int tid = __builtin_amdgcn_readfirstlane((int)e[5]);  // cached tree id, -1 below depth 2
-      const Mom mpar = lane < NP ? Mom{e[6 + lane], e[6 + NP + lane]} : Mom{0.0, 0.0};
+    while (have) {
+      double nx0 = 0.0, ny0 = 0.0, ndx = 0.0, ndy = 0.0, nlev = 0.0, ntid = 0.0;
+      Mom tpar{0.0, 0.0};
+      if (PREFETCH && sp > 0) {
+        const double* e = stack + (size_t)(sp - 1) * ES;
+        nx0 = e[0];
+        ny0 = e[1];
+        ndx = e[2];
+        ndy = e[3];
+        nlev = e[4];
+        ntid = e[5];
+        if (lane < NP) tpar = Mom{e[6 + lane], e[6 + NP + lane]};
+      }
+      const double x0 = cx0, y0 = cy0, dx = cdx, dy = cdy;
+      const int level = clevel, tid = ctid;  // cached tree id, -1 below the cached depths
+      const Mom mpar = cpar;
       if (per_tri) {
         if (tid >= 0 && tid < npar)
           step_tris<N, HELM, true>(pb, lsc, false, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
@@ -952,17 +964,18 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
         aborted = true;
         break;
       }
+      // (sp + 4: the capacity rule of a stack that holds all four children)
       if (refine && level + 1 <= NLMAX && sp + 4 <= STACK_CAP) {
         const double hx = 0.5 * dx, hy = 0.5 * dy;
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < 3; c++) {
           double* ne = stack + (size_t)(sp + c) * ES;
           if (lane == 0) {
             const int ct = cached_child(tid, c);
             ne[0] = (c & 1) ? x0 + hx : x0;
             ne[1] = (c & 2) ? y0 + hy : y0;
-            ne[2] = c == 3 ? -hx : hx;
-            ne[3] = c == 3 ? -hy : hy;
+            ne[2] = hx;
+            ne[3] = hy;
             ne[4] = (double)(level + 1);
             ne[5] = (double)(ct >= 0 && ct < NCACHE_PARENT ? ct : -1);
           }
@@ -971,11 +984,41 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
             ne[6 + NP + lane] = mc[c].im;
           }
         }
-        sp += 4;
+        sp += 3;
+        const int ct3 = cached_child(tid, 3);
+        cx0 = x0 + hx;
+        cy0 = y0 + hy;
+        cdx = -hx;
+        cdy = -hy;
+        clevel = level + 1;
+        ctid = ct3 >= 0 && ct3 < NCACHE_PARENT ? ct3 : -1;
+        cpar = lane < NP ? mc[3] : Mom{0.0, 0.0};
       } else {
         if (refine && lane == 0) atomicAdd(overflow, 1);  // level 20 or stack depth reached
         result += ssum;
         deep = level + 1 > deep ? level + 1 : deep;
+        if (sp > 0) {
+          sp--;
+          if (!PREFETCH) {
+            const double* e = stack + (size_t)sp * ES;
+            nx0 = e[0];
+            ny0 = e[1];
+            ndx = e[2];
+            ndy = e[3];
+            nlev = e[4];
+            ntid = e[5];
+            if (lane < NP) tpar = Mom{e[6 + lane], e[6 + NP + lane]};
+          }
+          cx0 = wave_uniform(nx0);
+          cy0 = wave_uniform(ny0);
+          cdx = wave_uniform(ndx);
+          cdy = wave_uniform(ndy);
+          clevel = __builtin_amdgcn_readfirstlane((int)nlev);
+          ctid = __builtin_amdgcn_readfirstlane((int)ntid);
+          cpar = tpar;
+        } else {
+          have = false;
+        }
       }
       __syncthreads();
     }
