@@ -607,11 +607,6 @@ __global__ void k_cplx_to_real(int64_t n, const V2<double>* __restrict__ a, doub
   if (i < n) b[i] = a[i].x;
 }
 
-__global__ void k_zero2(int64_t n, V2<double>* a) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) a[i] = {0.0, 0.0};
-}
-
 template <typename T>
 T* dal(size_t n) {
   void* p = nullptr;

@@ -102,6 +102,7 @@ __global__ void __launch_bounds__(256) k_far_order(int64_t npat, const int64_t* 
   __shared__ int s_done;
   const int tid = threadIdx.x;
   const int64_t ip = blockIdx.x;
+  if (ip >= npat) return;  // (the grid is one workgroup per patch)
   const double* sc = srccoefs + 9 * (ixyzs[ip] - 1);
   const double* cm = cms + 3 * ip;
   // probes: the farthest near targets (ties: higher index first), then sphere points

@@ -236,10 +236,9 @@ __global__ void k_scatter_fixed(int64_t ntarg, const int64_t* __restrict__ row_p
 }
 
 // ------------------------------------------------------------------ 3. fixed rule
-// Bq[q*NP + j] = w_q * sum_l P_l(q) umat(l,j); Pq[q*NP + l] = P_l(q); quv[2q] = (u,v).
+// Bq[q*NP + j] = w_q * sum_l P_l(q) umat(l,j); Pq[q*NP + l] = P_l(q).
 template <int N, bool HELM>
-__global__ void __launch_bounds__(128) k_fixed(int nf, const double* __restrict__ quv,
-                                               const double* __restrict__ Pq, const double* __restrict__ Bq,
+__global__ void __launch_bounds__(128) k_fixed(int nf, const double* __restrict__ Pq, const double* __restrict__ Bq,
                                                const int64_t* __restrict__ ixyzs,
                                                const double* __restrict__ srccoefs, int64_t ndtarg,
                                                const double* __restrict__ targs,
@@ -512,8 +511,6 @@ k_tree_cache_mfma(int64_t npatches, int ncn, const double* __restrict__ twq, con
   constexpr int PW = ptab_w(NP);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ia = lane & 15, ka = lane >> 4;
-  const int64_t p = (int64_t)blockIdx.x * 16 + ia;
-  const bool pok = p < npatches;
   // B fragments of the 16 patches in LDS: step s, component m, lane l at ((s 9 + m) 64 + l)
   __shared__ double bs[KS * 9 * 64];
   __shared__ __align__(16) double rec[4 * 512];  // per wavefront: 64 records of 8 doubles
@@ -525,7 +522,6 @@ k_tree_cache_mfma(int64_t npatches, int ncn, const double* __restrict__ twq, con
     bs[i] = (pi < npatches && l < NP) ? srccoefs[9 * (ixyzs[pi] - 1) + 9 * l + m] : 0.0;
   }
   __syncthreads();
-  double* out = cache + (size_t)(pok ? p : 0) * CACHE_STRIDE * ncn;
   const int ntile = (ncn + 15) / 16;
   const int tstep = 4 * gridDim.y;
   // A fragments of the next tile are loaded while the current one is multiplied
@@ -661,8 +657,7 @@ __device__ __forceinline__ Mom column_sum_rows(const double* pb, int r0, int r1)
 // coefficients sc in LDS.  CACHED selects the path at compile time (the caller tests the
 // step's tree id), so each path is scheduled without the other's registers.
 template <int N, bool HELM, bool CACHED>
-__device__ __forceinline__ void step_moments(double* pb, const double* sc, bool root, int npar,
-                                             int tid, double x0, double y0, double dx, double dy, int kq,
+__device__ __forceinline__ void step_moments(double* pb, const double* sc, bool root, int tid, double x0, double y0, double dx, double dy, int kq,
                                              const double* __restrict__ xg, const double* __restrict__ cpatch,
                                              const double* __restrict__ ptab, const double* tg, const KPar& kp,
                                              Mom (&m)[4]) {
@@ -910,7 +905,7 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
     if (per_tri)
       step_tris<N, HELM, true>(pb, lsc, true, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
     else
-      step_moments<N, HELM, true>(pb, lsc, true, npar, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
+      step_moments<N, HELM, true>(pb, lsc, true, 0, 0.0, 0.0, 1.0, 1.0, kq, xg, cpatch, ptab, tg, kp, mc);
     // The triangle being tested is held in registers and the stack holds the others:
     // a refinement pushes children 0..2 and goes on with child 3 (the one the stack
     // order takes next).  At order 6 and above the entry on top of the stack is loaded at
@@ -945,9 +940,9 @@ __global__ void __launch_bounds__(64) k_adaptive(int nlist, const int* __restric
         else
           step_tris<N, HELM, false>(pb, lsc, false, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
       } else if (tid >= 0 && tid < npar) {
-        step_moments<N, HELM, true>(pb, lsc, false, npar, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
+        step_moments<N, HELM, true>(pb, lsc, false, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
       } else {
-        step_moments<N, HELM, false>(pb, lsc, false, npar, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
+        step_moments<N, HELM, false>(pb, lsc, false, tid, x0, y0, dx, dy, kq, xg, cpatch, ptab, tg, kp, mc);
       }
       nref++;
       nunc += (tid < 0 || tid >= npar) ? 1 : 0;  // children below the cached tree
@@ -1372,7 +1367,7 @@ T* wsalloc(int slot, size_t n) {
 // Patch-independent tables of one (order, rule) combination, built once per device.
 struct OrderTables {
   int kq = 0, nf = 0;
-  double *umat, *rvw, *rvuv, *xg, *fuv, *Pq, *Bq, *twq, *ptab;
+  double *umat, *rvw, *rvuv, *xg, *Pq, *Bq, *twq, *ptab;
 };
 std::map<std::tuple<int, int, int, int, int>, OrderTables> g_tables;
 
@@ -1521,13 +1516,12 @@ int build_tables(int nqorder, int nlev, int nqorder_f, OrderTables& T, hipStream
   T.rvw = upload(rvw, s);
   T.rvuv = upload(rvuv, s);
   T.xg = upload(xg, s);
-  T.fuv = upload(fuv, s);
   T.Pq = upload(Pq, s);
   T.Bq = upload(Bq, s);
   T.twq = upload(twq, s);
   T.ptab = upload(ptab, s);
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  if (!T.umat || !T.rvw || !T.rvuv || !T.xg || !T.fuv || !T.Pq || !T.Bq || !T.twq || !T.ptab) return -2;
+  if (!T.umat || !T.rvw || !T.rvuv || !T.xg || !T.Pq || !T.Bq || !T.twq || !T.ptab) return -2;
   return 0;
 }
 
@@ -1565,7 +1559,6 @@ inline int class_degree(int q0, int c) {
   const int q = q0 + off[c];
   return q < 8 ? (q0 < 8 ? q0 : 8) : (q > 30 ? 30 : q);
 }
-constexpr int PLAIN_CLASS = 2;
 
 __global__ void k_seg_bounds(int nrules, int64_t npat, const int* __restrict__ boff, int* __restrict__ seg) {
   const int c = threadIdx.x;
@@ -1622,8 +1615,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
   }
   const OrderTables& TB = tit->second;
   const int kq = TB.kq, nf = TB.nf;
-  double *d_umat = TB.umat, *d_rvw = TB.rvw, *d_rvuv = TB.rvuv, *d_xg = TB.xg, *d_fuv = TB.fuv, *d_Pq = TB.Pq,
-         *d_Bq = TB.Bq;
+  double *d_umat = TB.umat, *d_rvw = TB.rvw, *d_rvuv = TB.rvuv, *d_xg = TB.xg, *d_Pq = TB.Pq, *d_Bq = TB.Bq;
   double *d_cms = wsalloc<double>(W_CMS, 3 * npat), *d_rads = wsalloc<double>(W_RADS, npat),
          *d_epsp = wsalloc<double>(W_EPSP, npat), *d_area = wsalloc<double>(W_AREA, npat);
   int *d_cnt = wsalloc<int>(W_CNT, npat), *d_off = wsalloc<int>(W_OFF, npat + 1),
@@ -1717,7 +1709,7 @@ int run_order(const NearQuadArgs& A, NearQuadStats* st, hipStream_t s) {
                                                         A.iquad, d_off, d_fixed, kp, A.wnear);
   } else {
     const size_t lds_fixed = sizeof(double) * (9 * NP + 7 * (size_t)nf);
-    k_fixed<N, HELM><<<npat, 128, lds_fixed, s2>>>(nf, d_fuv, d_Pq, d_Bq, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
+    k_fixed<N, HELM><<<npat, 128, lds_fixed, s2>>>(nf, d_Pq, d_Bq, A.ixyzs, A.srccoefs, A.ndtarg, A.targs,
                                                    A.iquad, d_off, d_fixed, kp, A.wnear);
   }
   HCHK(hipGetLastError());
